@@ -1,0 +1,259 @@
+"""Python mirror of the reference's algorithm classes over libtdhip.so.
+
+The reference's GPU classes (Treedepth/src/dynamic_gpu/dynamic_gpu.hpp:15-64,
+Treedepth/src/bnb_gpu/bnb_gpu.hpp:8-31) take a Boost graph; here they take a
+:class:`gisbdw_amd.graphs.Graph` and call the C ABI declared in
+include/tdhip.h through ctypes.  There is no CPU fallback: if the HIP library
+is missing or no GPU is visible, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .graphs import Graph
+
+_HERE = Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "libtdhip.so"
+_lib = None
+
+TD_OK, TD_E_INVALID, TD_E_NOMEM, TD_E_HIP, TD_E_BUDGET, TD_E_STATE = range(6)
+_STATUS = {
+    TD_E_INVALID: "TD_E_INVALID",
+    TD_E_NOMEM: "TD_E_NOMEM",
+    TD_E_HIP: "TD_E_HIP",
+    TD_E_BUDGET: "TD_E_BUDGET",
+    TD_E_STATE: "TD_E_STATE",
+}
+
+
+class TDError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class DPStats(C.Structure):
+    _fields_ = [
+        ("graphs", C.c_uint64),
+        ("subsets", C.c_uint64),
+        ("alg_bytes", C.c_uint64),
+        ("launches", C.c_uint64),
+        ("level_ms", C.c_double),
+        ("tree_ms", C.c_double),
+        ("total_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class BnBStats(C.Structure):
+    _fields_ = [
+        ("graphs", C.c_uint64),
+        ("certified", C.c_uint64),
+        ("nodes", C.c_uint64),
+        ("memo_hits", C.c_uint64),
+        ("kernel_ms", C.c_double),
+        ("total_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def library_path() -> Path:
+    return _LIB_PATH
+
+
+def load_library():
+    """Load libtdhip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        raise RuntimeError(
+            f"{_LIB_PATH} is missing: build it with `python -m gisbdw_amd.build` "
+            "(the HIP library is the only implementation; there is no fallback)")
+    lib = C.CDLL(str(_LIB_PATH))
+    P = C.POINTER
+    sig = {
+        "td_ctx_create": (C.c_int, [P(C.c_void_p), C.c_int, C.c_uint64]),
+        "td_ctx_destroy": (None, [C.c_void_p]),
+        "td_last_error": (C.c_char_p, [C.c_void_p]),
+        "td_ctx_set_param": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+        "td_ctx_get_param": (C.c_int, [C.c_void_p, C.c_char_p, P(C.c_int64)]),
+        "td_dp_solve": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_uint64), P(C.c_uint8),
+                                  P(C.c_int8), P(DPStats)]),
+        "td_dp_get_treedepth": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, P(C.c_uint32)]),
+        "td_dp_get_level": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_uint8), C.c_uint64]),
+        "td_dp_get_elimination": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_uint64,
+                                            P(C.c_int32)]),
+        "td_dp_solve_batch": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
+                                        P(C.c_uint8), P(C.c_int8), C.c_int32, P(DPStats)]),
+        "td_dp_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, P(C.c_double)]),
+    }
+    optional = {
+        "td_bnb_solve_batch": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
+                                         P(C.c_int32), P(C.c_float), P(C.c_uint8), P(C.c_int8),
+                                         P(C.c_uint8), P(BnBStats)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    for name, (res, args) in optional.items():
+        if hasattr(lib, name):
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Context:
+    """Owns a td_ctx (HIP stream, device arena, tuning parameters)."""
+
+    def __init__(self, device: int = 0, mem_budget_bytes: int = 0, **params):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.td_ctx_create(C.byref(h), device, mem_budget_bytes)
+        if rc != TD_OK:
+            raise TDError(rc, "td_ctx_create failed (no usable HIP device?)")
+        self.h = h
+        for k, v in params.items():
+            self.set_param(k.replace("_", ".", 1), v)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.td_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def check(self, rc: int):
+        if rc != TD_OK:
+            raise TDError(rc, self.lib.td_last_error(self.h).decode())
+
+    def set_param(self, key: str, value: int):
+        self.check(self.lib.td_ctx_set_param(self.h, key.encode(), int(value)))
+
+    def get_param(self, key: str) -> int:
+        v = C.c_int64()
+        self.check(self.lib.td_ctx_get_param(self.h, key.encode(), C.byref(v)))
+        return v.value
+
+
+@dataclass
+class Result:
+    """EliminationTree::Result (Treedepth/src/elimination_tree/
+    elimination_tree.hpp:26-30): decomposition edges, treedepth, root.  For a
+    forest (disconnected input) ``root`` is the root of the tree holding
+    vertex 0 and ``roots`` lists them all."""
+
+    td_decomp: List[Tuple[int, int]]
+    treedepth: int
+    root: int
+    parent: List[int]
+    roots: List[int]
+
+    @classmethod
+    def from_parent(cls, parent: Sequence[int], treedepth: int) -> "Result":
+        parent = [int(p) for p in parent]
+        edges = sorted((min(v, p), max(v, p)) for v, p in enumerate(parent) if p >= 0)
+        roots = [v for v, p in enumerate(parent) if p < 0]
+        r = 0
+        while parent[r] >= 0:
+            r = parent[r]
+        return cls(edges, int(treedepth), r, parent, roots)
+
+
+class DynamicGPU:
+    """td::DynamicGPU (dynamic_gpu.hpp:15-36) on the MI355X subset DP.
+
+    ``dp(g)`` runs the DP (operator()); ``GetElimination(n, k, code)`` and
+    ``GetTreedepth(n, k, code)`` query the level tables left on the device.
+    """
+
+    def __init__(self, ctx: Optional[Context] = None):
+        self.ctx = ctx or Context()
+        self.lib = self.ctx.lib
+        self.n = 0
+        self.treedepth = None
+        self.parent = None
+        self.stats = DPStats()
+
+    def __call__(self, g: Graph) -> "DynamicGPU":
+        adj = g.adj_array()
+        td = np.zeros(1, np.uint8)
+        par = np.zeros(max(g.n, 1), np.int8)
+        self.ctx.check(self.lib.td_dp_solve(self.ctx.h, g.n, _ptr(adj, C.c_uint64),
+                                            _ptr(td, C.c_uint8), _ptr(par, C.c_int8),
+                                            C.byref(self.stats)))
+        self.n = g.n
+        self.treedepth = int(td[0])
+        self.parent = par[: g.n].astype(int).tolist()
+        return self
+
+    def result(self) -> Result:
+        return Result.from_parent(self.parent, self.treedepth)
+
+    def GetElimination(self, nverts: int, subset_size: int, subset_code: int) -> List[int]:
+        out = np.zeros(max(subset_size, 1), np.int32)
+        self.ctx.check(self.lib.td_dp_get_elimination(self.ctx.h, nverts, subset_size,
+                                                      subset_code, _ptr(out, C.c_int32)))
+        return out[:subset_size].tolist()
+
+    def GetTreedepth(self, nverts: int, subset_size: int, subset_code: int) -> int:
+        if nverts != self.n:
+            raise TDError(TD_E_INVALID, "nverts differs from the solved graph")
+        v = C.c_uint32()
+        self.ctx.check(self.lib.td_dp_get_treedepth(self.ctx.h, subset_size, subset_code,
+                                                    C.byref(v)))
+        return v.value
+
+    def level(self, k: int) -> np.ndarray:
+        from math import comb
+
+        out = np.zeros(max(comb(self.n, k), 1), np.uint8)
+        self.ctx.check(self.lib.td_dp_get_level(self.ctx.h, k, _ptr(out, C.c_uint8), out.size))
+        return out[: comb(self.n, k)]
+
+
+def dp_solve_batch(ctx: Context, graphs: Sequence[Graph], mode: int = 0):
+    """td::DynamicGPU over many graphs in one call.  Returns (treedepths,
+    parents, stats)."""
+    lib = ctx.lib
+    ns = np.array([g.n for g in graphs], np.int32)
+    adj = np.concatenate([g.adj_array() for g in graphs]).astype(np.uint64)
+    td = np.zeros(len(graphs), np.uint8)
+    par = np.zeros(int(ns.sum()), np.int8)
+    st = DPStats()
+    ctx.check(lib.td_dp_solve_batch(ctx.h, len(graphs), _ptr(ns, C.c_int32),
+                                    _ptr(adj, C.c_uint64), _ptr(td, C.c_uint8),
+                                    _ptr(par, C.c_int8), mode, C.byref(st)))
+    parents = []
+    off = 0
+    for g in graphs:
+        parents.append(par[off: off + g.n].astype(int).tolist())
+        off += g.n
+    return td.astype(int).tolist(), parents, st

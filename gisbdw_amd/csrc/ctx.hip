@@ -1,0 +1,202 @@
+// ctx.hip -- context lifetime, device buffers, parameters and host-side
+// combinatorics shared by the DP and the branch-and-bound.
+#include <cstring>
+
+#include "ctx.hpp"
+
+namespace tdhip {
+
+uint64_t Binom(int n, int k) {
+  static uint64_t table[65][65];
+  static bool init = false;
+  if (!init) {
+    for (int i = 0; i <= 64; ++i) {
+      table[i][0] = 1;
+      for (int j = 1; j <= 64; ++j)
+        table[i][j] = (i == 0) ? 0 : table[i - 1][j - 1] + table[i - 1][j];
+    }
+    init = true;
+  }
+  if (n < 0 || k < 0 || k > n) return 0;
+  return table[n][k];
+}
+
+// colex: rank(S) = sum_i C(s_i, i+1) over the sorted elements s_0 < s_1 < ...
+uint64_t RankColex(uint64_t mask) {
+  uint64_t r = 0;
+  int i = 0;
+  while (mask) {
+    int v = __builtin_ctzll(mask);
+    mask &= mask - 1;
+    r += Binom(v, ++i);
+  }
+  return r;
+}
+
+uint64_t UnrankColex(int n, int k, uint64_t code) {
+  uint64_t m = 0;
+  for (int v = n - 1; v >= 0 && k > 0; --v) {
+    uint64_t c = Binom(v, k);
+    if (code >= c) {
+      m |= 1ull << v;
+      code -= c;
+      --k;
+    }
+  }
+  return m;
+}
+
+void ValidateGraph(int n, const uint64_t* adj, int max_n) {
+  TD_REQUIRE(n >= 1 && n <= max_n, TD_E_INVALID,
+             "n must be in [1, " + std::to_string(max_n) + "]");
+  TD_REQUIRE(adj != nullptr, TD_E_INVALID, "adjacency is null");
+  uint64_t full = n >= 64 ? ~0ull : ((1ull << n) - 1);
+  for (int v = 0; v < n; ++v) {
+    TD_REQUIRE(!(adj[v] & ~full), TD_E_INVALID,
+               "row " + std::to_string(v) + " names a vertex >= n");
+    TD_REQUIRE(!(adj[v] >> v & 1), TD_E_INVALID,
+               "self loop at vertex " + std::to_string(v));
+    for (uint64_t m = adj[v]; m; m &= m - 1) {
+      int u = __builtin_ctzll(m);
+      TD_REQUIRE(adj[u] >> v & 1, TD_E_INVALID,
+                 "asymmetric edge " + std::to_string(v) + "-" +
+                     std::to_string(u));
+    }
+  }
+}
+
+}  // namespace tdhip
+
+void* td_ctx::Buffer(const std::string& name, size_t bytes) {
+  auto& b = bufs[name];
+  if (b.bytes >= bytes && b.p) return b.p;
+  if (b.p) {
+    TD_HIP(hipStreamSynchronize(stream));
+    TD_HIP(hipFree(b.p));
+    allocated -= b.bytes;
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  if (budget && allocated + bytes > budget)
+    throw tdhip::Error{TD_E_NOMEM, "buffer '" + name + "' of " +
+                                       std::to_string(bytes) +
+                                       " bytes exceeds the context budget"};
+  hipError_t e = hipMalloc(&b.p, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    throw tdhip::Error{TD_E_NOMEM, "hipMalloc of " + std::to_string(bytes) +
+                                       " bytes for '" + name +
+                                       "' failed: " + hipGetErrorString(e)};
+  }
+  b.bytes = bytes;
+  allocated += bytes;
+  return b.p;
+}
+
+void td_ctx::Release(const std::string& name) {
+  auto it = bufs.find(name);
+  if (it == bufs.end()) return;
+  if (it->second.p) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(it->second.p);
+    allocated -= it->second.bytes;
+  }
+  bufs.erase(it);
+}
+
+extern "C" {
+
+int td_ctx_create(td_ctx** out, int device, uint64_t mem_budget_bytes) {
+  if (!out) return TD_E_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    (void)hipGetLastError();
+    return TD_E_HIP;
+  }
+  if (device < 0 || device >= count) return TD_E_INVALID;
+  td_ctx* c = new td_ctx();
+  c->device = device;
+  c->budget = mem_budget_bytes;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipGetDeviceProperties(&c->prop, device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return TD_E_HIP;
+  }
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete c;
+      return TD_E_HIP;
+    }
+  *out = c;
+  return TD_OK;
+}
+
+void td_ctx_destroy(td_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->dp_graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : c->bufs)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* td_last_error(const td_ctx* c) {
+  return c ? c->err.c_str() : "null context";
+}
+
+int td_ctx_set_param(td_ctx* c, const char* key, int64_t v) {
+  if (!c || !key) return TD_E_INVALID;
+  std::string k(key);
+  auto bad = [&](const char* why) {
+    c->err = k + ": " + why;
+    return TD_E_INVALID;
+  };
+  if (k == "dp.low_bits") {
+    if (v < 2 || v > 5) return bad("must be in [2, 5]");
+    c->dp.low_bits = (int)v;
+  } else if (k == "dp.tiles_per_lane") {
+    if (v < 1 || v > 64) return bad("must be in [1, 64]");
+    c->dp.tiles_per_lane = (int)v;
+  } else if (k == "dp.block") {
+    if (v < 64 || v > 1024 || v % 64) return bad("must be a multiple of 64 in [64, 1024]");
+    c->dp.block = (int)v;
+  } else if (k == "dp.batch_wg_per_cu") {
+    if (v < 1 || v > 32) return bad("must be in [1, 32]");
+    c->dp.batch_wg_per_cu = (int)v;
+  } else if (k == "dp.use_graph") {
+    c->dp.use_graph = v ? 1 : 0;
+  } else if (k.rfind("bnb.", 0) == 0 || k.rfind("mlp.", 0) == 0) {
+    c->params[k] = v;
+  } else {
+    return bad("unknown parameter");
+  }
+  // a change of DP launch shape invalidates the captured level graphs
+  for (auto& kv : c->dp_graphs) (void)hipGraphExecDestroy(kv.second);
+  c->dp_graphs.clear();
+  return TD_OK;
+}
+
+int td_ctx_get_param(const td_ctx* c, const char* key, int64_t* v) {
+  if (!c || !key || !v) return TD_E_INVALID;
+  std::string k(key);
+  if (k == "dp.low_bits") *v = c->dp.low_bits;
+  else if (k == "dp.tiles_per_lane") *v = c->dp.tiles_per_lane;
+  else if (k == "dp.block") *v = c->dp.block;
+  else if (k == "dp.batch_wg_per_cu") *v = c->dp.batch_wg_per_cu;
+  else if (k == "dp.use_graph") *v = c->dp.use_graph;
+  else {
+    auto it = c->params.find(k);
+    if (it == c->params.end()) return TD_E_INVALID;
+    *v = it->second;
+  }
+  return TD_OK;
+}
+
+}  // extern "C"

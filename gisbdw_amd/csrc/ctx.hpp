@@ -1,0 +1,110 @@
+// ctx.hpp -- the td_ctx object behind include/tdhip.h and shared helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/tdhip.h"
+
+namespace tdhip {
+
+// Error reporting: every entry point returns a td_status and leaves a
+// message in ctx->err.  HIP failures are turned into TD_E_HIP here.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define TD_HIP(call)                                                      \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess)                                                 \
+      throw ::tdhip::Error{TD_E_HIP, std::string(#call) + ": " +          \
+                                         hipGetErrorString(e_)};          \
+  } while (0)
+
+#define TD_REQUIRE(cond, code, msg)                  \
+  do {                                               \
+    if (!(cond)) throw ::tdhip::Error{(code), (msg)}; \
+  } while (0)
+
+// Per-graph DP descriptor, one per graph, read by the kernels with scalar
+// loads.  Vertices 0..lb-1 are the "low" vertices resolved inside a tile;
+// vertices lb..n-1 are the "high" vertices whose subset H selects the tile.
+struct DPGraph {
+  uint32_t n;        // vertices
+  uint32_t nh;       // high vertices, max(n - lb, 0)
+  uint32_t lb;       // low bits per tile
+  uint32_t pad0;
+  uint64_t table;    // byte offset of this graph's tables in the arena
+  uint32_t lvl[34];  // tile offset of high level h; lvl[nh+1] = 2^nh
+  uint32_t adj[32];  // adjacency rows
+  uint32_t nl[32];   // N(L) for every L subset of the low vertices
+  uint32_t nbh[512]; // N(H) by 7-bit chunks of H: nbh[128 c + chunk]
+};
+
+// DP launch configuration (autotuned).
+struct DPConfig {
+  int low_bits = 4;        // 2..5
+  int tiles_per_lane = 4;  // consecutive tiles per lane (Gosper steps)
+  int block = 256;         // threads per workgroup
+  int batch_wg_per_cu = 4; // persistent batch kernel occupancy target
+  int use_graph = 1;       // capture the level sequence into a hipGraph
+};
+
+struct DeviceBuffer {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace tdhip
+
+struct td_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0;
+  uint64_t allocated = 0;
+  std::string err;
+  hipDeviceProp_t prop{};
+  tdhip::DPConfig dp;
+  std::map<std::string, int64_t> params;
+
+  // named device buffers (grown on demand, freed with the context)
+  std::map<std::string, tdhip::DeviceBuffer> bufs;
+  void* Buffer(const std::string& name, size_t bytes);
+  void Release(const std::string& name);
+
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+
+  // state of the last td_dp_solve (for the queries)
+  int last_n = 0;
+  bool have_dp = false;
+  tdhip::DPGraph last_graph{};
+
+  // cached hipGraph per (n, config) for the single-graph level sequence
+  struct GraphKey {
+    int n, lb, r, block;
+    bool operator<(const GraphKey& o) const {
+      if (n != o.n) return n < o.n;
+      if (lb != o.lb) return lb < o.lb;
+      if (r != o.r) return r < o.r;
+      return block < o.block;
+    }
+  };
+  std::map<GraphKey, hipGraphExec_t> dp_graphs;
+  // buffer addresses baked into each captured graph (arena, descriptor)
+  std::map<GraphKey, std::pair<const void*, const void*>> dp_graph_ptrs;
+};
+
+namespace tdhip {
+// binomial table C(i, j), i, j <= 64, host side
+uint64_t Binom(int n, int k);
+// colex unrank on the host (n vertices, k-subset)
+uint64_t UnrankColex(int n, int k, uint64_t code);
+uint64_t RankColex(uint64_t mask);
+// validate a graph; returns the adjacency as 32-bit rows when n <= 32
+void ValidateGraph(int n, const uint64_t* adj, int max_n);
+}  // namespace tdhip

@@ -1,0 +1,962 @@
+// dp.hip -- subset dynamic program for exact treedepth on gfx950.
+//
+// Reference path: DynamicGPU (Treedepth/src/dynamic_gpu/dynamic_gpu.cu:40-201,
+// dynamic_gpu.hpp:38-64).  Recurrence (docs introduction/main.tex:10-15):
+//   td(S) = 1 + min_{v in S} td(S \ v)   if S is connected,
+//   td(S) = max over the components of S otherwise.
+// For a disconnected S the second line equals max_{v in S} td(S \ v), so every
+// subset of size k needs only subsets of size k-1.  Connectivity is carried in
+// the table too: S (|S| >= 2) is connected iff no vertex of S is isolated in S
+// and some S \ v is connected.  Each stored byte is 2*td + connected.
+//
+// Layout.  Vertices 0..lb-1 are "low", lb..n-1 "high".  A subset S = H u L is
+// stored in the tile of its high part H (2^lb bytes, byte L).  The tiles of
+// high level h = |H| are ranked by the colex rank of H among the h-subsets of
+// the nh = n - lb high vertices (rank(H) = sum_i C(x_i, i+1),
+// set_encoder.hpp:40-46), and the levels are concatenated.  The tile of H is
+// computed from the h tiles of H \ x (x in H, all at high level h-1, each a
+// contiguous 2^lb-byte block read once) and, inside the tile, from smaller L
+// of the same tile (registers).  One lane owns a tile; a lane walks
+// consecutive ranks with Gosper's successor (colex order = numeric order).
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace tdhip {
+namespace {
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+constexpr int kCbStride = 33;  // s_cb[x * 33 + j] = C(x, j), x < 32, j <= 32
+
+__device__ __forceinline__ u16x2 AsU16x2(uint32_t x) {
+  return __builtin_bit_cast(u16x2, x);
+}
+
+// Colex unrank of r among the k-subsets of {0..nh-1}; greedy from the top
+// (set_encoder.hpp:61-73) with the binomials read from LDS.
+__device__ __forceinline__ uint32_t Unrank(uint32_t r, int k, int nh,
+                                           const uint32_t* s_cb) {
+  uint32_t H = 0;
+  for (int x = nh - 1; x >= 0; --x) {
+    const uint32_t c = s_cb[x * kCbStride + k];
+    if (k > 0 && r >= c) {
+      H |= 1u << x;
+      r -= c;
+      --k;
+    }
+  }
+  return H;
+}
+
+// Next k-subset in colex order (Gosper's hack, shift instead of divide).
+__device__ __forceinline__ uint32_t NextCombination(uint32_t x) {
+  const uint32_t c = x & (0u - x);
+  const uint32_t r = x + c;
+  return r | (((x ^ r) >> 2) >> __builtin_ctz(x));
+}
+
+template <int LB>
+__device__ __forceinline__ void LoadTile(const uint8_t* p,
+                                         uint32_t (&w)[(1 << LB) / 4]) {
+  if constexpr (LB == 2) {
+    w[0] = *reinterpret_cast<const uint32_t*>(p);
+  } else if constexpr (LB == 3) {
+    const uint2 v = *reinterpret_cast<const uint2*>(p);
+    w[0] = v.x;
+    w[1] = v.y;
+  } else {
+#pragma unroll
+    for (int i = 0; i < (1 << LB) / 16; ++i) {
+      const uint4 v = reinterpret_cast<const uint4*>(p)[i];
+      w[4 * i + 0] = v.x;
+      w[4 * i + 1] = v.y;
+      w[4 * i + 2] = v.z;
+      w[4 * i + 3] = v.w;
+    }
+  }
+}
+
+template <int LB>
+__device__ __forceinline__ void StoreTile(uint8_t* p,
+                                          const uint32_t (&w)[(1 << LB) / 4]) {
+  if constexpr (LB == 2) {
+    *reinterpret_cast<uint32_t*>(p) = w[0];
+  } else if constexpr (LB == 3) {
+    *reinterpret_cast<uint2*>(p) = make_uint2(w[0], w[1]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < (1 << LB) / 16; ++i)
+      reinterpret_cast<uint4*>(p)[i] =
+          make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+  }
+}
+
+// Resolve one tile: all 2^LB subsets H u L of high part H (|H| = h, colex
+// rank t among high level h).  prev = first tile of high level h-1, cur =
+// first tile of high level h.
+template <int LB>
+__device__ __forceinline__ void DPTile(const uint8_t* __restrict__ prev,
+                                       uint8_t* __restrict__ cur, uint32_t t,
+                                       uint32_t H, int h,
+                                       const uint32_t* s_cb,
+                                       const uint32_t* s_nbh,
+                                       const uint32_t (&nl)[1 << LB]) {
+  constexpr int NB = 1 << LB;
+  constexpr int NW = NB / 4;
+  // byte-wise min / max of the neighbour tiles, kept as u16 pairs: E holds
+  // bytes 0 and 2 of each dword, O bytes 1 and 3; orr ORs raw bytes (bit 0
+  // of each byte = "some S \ x is connected").
+  u16x2 mnE[NW], mnO[NW], mxE[NW], mxO[NW];
+  uint32_t orr[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    mnE[w] = mnO[w] = AsU16x2(0x007F007Fu);
+    mxE[w] = mxO[w] = AsU16x2(0u);
+    orr[w] = 0u;
+  }
+  // neighbour tiles H \ x_j, j = h-1 .. 0 (x_j = j-th smallest of H):
+  // rank(H \ x_j) = t - C(x_j, j+1) - sum_{i>j} (C(x_i, i+1) - C(x_i, i)).
+  uint32_t rem = H;
+  uint32_t A = 0;
+  for (int j = h - 1; j >= 0; --j) {
+    const int x = 31 - __builtin_clz(rem);
+    rem ^= 1u << x;
+    const uint32_t c1 = s_cb[x * kCbStride + j + 1];
+    const uint32_t c0 = s_cb[x * kCbStride + j];
+    const uint32_t r2 = t - c1 - A;
+    A += c1 - c0;
+    uint32_t wv[NW];
+    LoadTile<LB>(prev + (static_cast<size_t>(r2) << LB), wv);
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t x = wv[w];
+      const u16x2 xe = AsU16x2(__builtin_amdgcn_perm(x, x, 0x0c020c00u));
+      const u16x2 xo = AsU16x2(__builtin_amdgcn_perm(x, x, 0x0c030c01u));
+      mnE[w] = __builtin_elementwise_min(mnE[w], xe);
+      mnO[w] = __builtin_elementwise_min(mnO[w], xo);
+      mxE[w] = __builtin_elementwise_max(mxE[w], xe);
+      mxO[w] = __builtin_elementwise_max(mxO[w], xo);
+      orr[w] |= x;
+    }
+  }
+  // N(H) from the 7-bit chunk tables
+  const uint32_t NH = s_nbh[H & 127u] | s_nbh[128 + ((H >> 7) & 127u)] |
+                      s_nbh[256 + ((H >> 14) & 127u)] |
+                      s_nbh[384 + ((H >> 21) & 127u)];
+  const uint32_t Sh = H << LB;
+  uint32_t V[NB];
+#pragma unroll
+  for (int L = 0; L < NB; ++L) {
+    const int w = L >> 2;
+    const u16x2 em = (L & 1) ? mnO[w] : mnE[w];
+    const u16x2 eM = (L & 1) ? mxO[w] : mxE[w];
+    uint32_t m = (L & 2) ? em.y : em.x;
+    uint32_t M = (L & 2) ? eM.y : eM.x;
+    uint32_t o = orr[w] >> (8 * (L & 3));
+#pragma unroll
+    for (int l = 0; l < LB; ++l) {
+      if (L >> l & 1) {
+        const uint32_t v = V[L ^ (1 << l)];
+        m = min(m, v);
+        M = max(M, v);
+        o |= v;
+      }
+    }
+    const uint32_t S = Sh | static_cast<uint32_t>(L);
+    const uint32_t cover = NH | nl[L];
+    const bool conn = ((S & ~cover) == 0u) && (o & 1u);
+    uint32_t val = conn ? ((m & ~1u) + 3u) : (M & ~1u);
+    if (__builtin_popcount(L) + h == 1) val = 3u;  // a single vertex
+    V[L] = val;
+  }
+  uint32_t ow[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w)
+    ow[w] = V[4 * w] | (V[4 * w + 1] << 8) | (V[4 * w + 2] << 16) |
+            (V[4 * w + 3] << 24);
+  StoreTile<LB>(cur + (static_cast<size_t>(t) << LB), ow);
+}
+
+template <int LB>
+__device__ __forceinline__ void LoadShared(const DPGraph* __restrict__ g,
+                                           const uint32_t* __restrict__ g_cb,
+                                           uint32_t* s_cb, uint32_t* s_nbh,
+                                           bool load_cb) {
+  if (load_cb)
+    for (int i = threadIdx.x; i < 32 * kCbStride; i += blockDim.x)
+      s_cb[i] = g_cb[i];
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) s_nbh[i] = g->nbh[i];
+}
+
+// Tiles [first, end) of high level h of graph g, R consecutive per lane,
+// lanes of the launch striding by `stride` tiles.
+template <int LB>
+__device__ __forceinline__ void DPLevelRange(const DPGraph* __restrict__ g,
+                                             uint8_t* __restrict__ arena,
+                                             int h, uint32_t first,
+                                             uint32_t end, uint32_t lane_id,
+                                             uint32_t stride, int R,
+                                             const uint32_t* s_cb,
+                                             const uint32_t* s_nbh) {
+  uint32_t nl[1 << LB];
+#pragma unroll
+  for (int L = 0; L < (1 << LB); ++L) nl[L] = g->nl[L];
+  const int nh = static_cast<int>(g->nh);
+  uint8_t* base = arena + g->table;
+  const uint8_t* prev =
+      h > 0 ? base + (static_cast<size_t>(g->lvl[h - 1]) << LB) : base;
+  uint8_t* cur = base + (static_cast<size_t>(g->lvl[h]) << LB);
+  for (uint32_t t0 = first + lane_id * R; t0 < end; t0 += stride) {
+    uint32_t H = Unrank(t0, h, nh, s_cb);
+    const uint32_t tend = min(t0 + static_cast<uint32_t>(R), end);
+    for (uint32_t t = t0;;) {
+      DPTile<LB>(prev, cur, t, H, h, s_cb, s_nbh, nl);
+      if (++t >= tend) break;
+      H = NextCombination(H);
+    }
+  }
+}
+
+// One high level of one graph.
+template <int LB>
+__global__ __launch_bounds__(1024) void DPLevelKernel(
+    const DPGraph* __restrict__ g, const uint32_t* __restrict__ g_cb,
+    uint8_t* __restrict__ arena, int h, uint32_t ntiles, int R) {
+  __shared__ uint32_t s_cb[32 * kCbStride];
+  __shared__ uint32_t s_nbh[512];
+  LoadShared<LB>(g, g_cb, s_cb, s_nbh, true);
+  __syncthreads();
+  const uint32_t lane_id = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t stride = gridDim.x * blockDim.x * R;
+  DPLevelRange<LB>(g, arena, h, 0, ntiles, lane_id, stride, R, s_cb, s_nbh);
+}
+
+// Level-synchronous batch: one launch per high level h over all graphs;
+// each workgroup resolves one chunk (graph, tile range) of that level.
+struct Chunk {
+  uint32_t gi, first, end, pad;
+};
+
+template <int LB>
+__global__ __launch_bounds__(1024) void DPChunkKernel(
+    const DPGraph* __restrict__ gs, const uint32_t* __restrict__ g_cb,
+    uint8_t* __restrict__ arena, const Chunk* __restrict__ chunks, int h,
+    int R) {
+  __shared__ uint32_t s_cb[32 * kCbStride];
+  __shared__ uint32_t s_nbh[512];
+  const Chunk c = chunks[blockIdx.x];
+  const DPGraph* g = gs + c.gi;
+  LoadShared<LB>(g, g_cb, s_cb, s_nbh, true);
+  __syncthreads();
+  DPLevelRange<LB>(g, arena, h, c.first, c.end, threadIdx.x,
+                   blockDim.x * R, R, s_cb, s_nbh);
+}
+
+// Persistent batch: workgroups pull whole graphs from a device queue (graphs
+// pre-sorted largest first) and resolve every level of a graph with a
+// workgroup barrier between levels, so no level of a graph ever waits on
+// another workgroup and the graph's tables stay in one XCD's L2.
+template <int LB>
+__global__ __launch_bounds__(256) void DPPersistentKernel(
+    const DPGraph* __restrict__ gs, const uint32_t* __restrict__ g_cb,
+    uint8_t* __restrict__ arena, int ngraphs, uint32_t* __restrict__ queue,
+    int R) {
+  __shared__ uint32_t s_cb[32 * kCbStride];
+  __shared__ uint32_t s_nbh[512];
+  __shared__ int s_gi;
+  for (int i = threadIdx.x; i < 32 * kCbStride; i += blockDim.x)
+    s_cb[i] = g_cb[i];
+  for (;;) {
+    if (threadIdx.x == 0) s_gi = static_cast<int>(atomicAdd(queue, 1u));
+    __syncthreads();
+    const int gi = s_gi;
+    if (gi >= ngraphs) break;  // uniform: every thread read the same s_gi
+    const DPGraph* g = gs + gi;
+    LoadShared<LB>(g, g_cb, s_cb, s_nbh, false);
+    __syncthreads();
+    const int nh = static_cast<int>(g->nh);
+    for (int h = 0; h <= nh; ++h) {
+      const uint32_t ntiles = g->lvl[h + 1] - g->lvl[h];
+      DPLevelRange<LB>(g, arena, h, 0, ntiles, threadIdx.x, blockDim.x * R, R,
+                       s_cb, s_nbh);
+      __syncthreads();
+    }
+  }
+}
+
+__global__ void ResetCounter(uint32_t* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
+}
+
+// td of subset S (raw byte >> 1) of graph g.
+__device__ __forceinline__ uint32_t Lookup(const DPGraph* __restrict__ g,
+                                           const uint8_t* __restrict__ arena,
+                                           const uint32_t* s_cb, uint32_t S) {
+  const uint32_t lb = g->lb;
+  const uint32_t L = S & ((1u << lb) - 1u);
+  uint32_t H = S >> lb;
+  const int h = __builtin_popcount(H);
+  uint32_t r = 0;
+  int i = 0;
+  while (H) {
+    const int x = __builtin_ctz(H);
+    H &= H - 1;
+    r += s_cb[x * kCbStride + (++i)];
+  }
+  const size_t addr =
+      g->table + ((static_cast<size_t>(g->lvl[h]) + r) << lb) + L;
+  return static_cast<uint32_t>(arena[addr]) >> 1;
+}
+
+__device__ __forceinline__ uint32_t WaveMax(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+  return v;
+}
+
+// GetElimination (dynamic_gpu.hpp:49-64) for one start set per wave: pop the
+// root stored for the current set and repeat.  The root of a connected C is
+// the smallest v with td(C \ v) = td(C) - 1 (the strict < over ascending v of
+// dynamic_gpu.cu:84-87); a disconnected set stores the root of its first
+// component (lowest vertex order) of largest td (dynamic_gpu.cu:90-104).
+__global__ __launch_bounds__(64) void DPWalkKernel(
+    const DPGraph* __restrict__ gs, const uint32_t* __restrict__ g_cb,
+    const uint8_t* __restrict__ arena, const uint32_t* __restrict__ start,
+    const int32_t* __restrict__ nsteps, int32_t* __restrict__ orders,
+    int order_stride) {
+  __shared__ uint32_t s_cb[32 * kCbStride];
+  for (int i = threadIdx.x; i < 32 * kCbStride; i += blockDim.x)
+    s_cb[i] = g_cb[i];
+  __syncthreads();
+  const DPGraph* g = gs + blockIdx.x;
+  const int lane = threadIdx.x;
+  uint32_t S = start[blockIdx.x];
+  const int steps = nsteps[blockIdx.x];
+  int32_t* out = orders + static_cast<size_t>(blockIdx.x) * order_stride;
+  for (int step = 0; step < steps && S != 0u; ++step) {
+    // components of S, lowest vertex first (uniform scalar work)
+    uint32_t rest = S, mine = 0;
+    int ncomp = 0;
+    while (rest) {
+      uint32_t comp = rest & (0u - rest), frontier = comp;
+      while (frontier) {
+        const int u = __builtin_ctz(frontier);
+        frontier &= frontier - 1;
+        const uint32_t nb = g->adj[u] & rest & ~comp;
+        comp |= nb;
+        frontier |= nb;
+      }
+      if (lane == ncomp) mine = comp;
+      rest &= ~comp;
+      ++ncomp;
+    }
+    const uint32_t tdc = lane < ncomp ? Lookup(g, arena, s_cb, mine) : 0u;
+    const uint32_t T = WaveMax(tdc);
+    const unsigned long long cb = __ballot(lane < ncomp && tdc == T);
+    const int ci = __ffsll(static_cast<long long>(cb)) - 1;
+    const uint32_t C = __shfl(mine, ci);
+    const bool in = lane < 32 && (C >> lane & 1u);
+    const uint32_t tv = in ? Lookup(g, arena, s_cb, C & ~(1u << lane)) : 0xFFu;
+    const unsigned long long rb = __ballot(in && tv + 1u == T);
+    const int root = __ffsll(static_cast<long long>(rb)) - 1;
+    if (lane == 0) out[step] = root;
+    S &= ~(1u << root);
+  }
+}
+
+// Level-k table in colex order over n vertices (td values).
+__global__ __launch_bounds__(256) void DPExportLevelKernel(
+    const DPGraph* __restrict__ g, const uint32_t* __restrict__ g_cb,
+    const uint8_t* __restrict__ arena, int k, uint32_t count,
+    uint8_t* __restrict__ out) {
+  __shared__ uint32_t s_cb[32 * kCbStride];
+  for (int i = threadIdx.x; i < 32 * kCbStride; i += blockDim.x)
+    s_cb[i] = g_cb[i];
+  __syncthreads();
+  const int n = static_cast<int>(g->n);
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < count;
+       r += gridDim.x * blockDim.x) {
+    const uint32_t S = Unrank(r, k, n, s_cb);
+    out[r] = static_cast<uint8_t>(Lookup(g, arena, s_cb, S));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+constexpr int kMaxDPN = 32;
+constexpr int kMaxBatchN = 26;
+
+// Tile offsets of each high level, each level padded to a 256-byte boundary
+// so no cache line holds tiles of two levels.
+DPGraph MakeGraph(int n, const uint64_t* adj, int lb, uint64_t table_off) {
+  DPGraph G;
+  std::memset(&G, 0, sizeof(G));
+  G.n = n;
+  G.lb = lb;
+  G.nh = n > lb ? n - lb : 0;
+  G.table = table_off;
+  const uint32_t pad_tiles = std::max(1u, 256u >> lb);
+  uint64_t off = 0;
+  for (uint32_t h = 0; h <= G.nh; ++h) {
+    G.lvl[h] = static_cast<uint32_t>(off);
+    off += Binom(G.nh, h);
+    off = (off + pad_tiles - 1) / pad_tiles * pad_tiles;
+  }
+  G.lvl[G.nh + 1] = static_cast<uint32_t>(off);
+  for (int v = 0; v < n; ++v) G.adj[v] = static_cast<uint32_t>(adj[v]);
+  for (int L = 0; L < (1 << lb); ++L) {
+    uint32_t m = 0;
+    for (int l = 0; l < lb && l < n; ++l)
+      if (L >> l & 1) m |= G.adj[l];
+    G.nl[L] = m;
+  }
+  for (int c = 0; c < 4; ++c)
+    for (int chunk = 0; chunk < 128; ++chunk) {
+      uint32_t m = 0;
+      for (int b = 0; b < 7; ++b) {
+        const int x = 7 * c + b;
+        if ((chunk >> b & 1) && x < static_cast<int>(G.nh)) m |= G.adj[lb + x];
+      }
+      G.nbh[128 * c + chunk] = m;
+    }
+  return G;
+}
+
+uint64_t TableBytes(const DPGraph& G) {
+  return static_cast<uint64_t>(G.lvl[G.nh + 1]) << G.lb;
+}
+
+const uint32_t* PascalDevice(td_ctx* ctx) {
+  auto it = ctx->bufs.find("dp.pascal");
+  if (it != ctx->bufs.end() && it->second.p)
+    return static_cast<const uint32_t*>(it->second.p);
+  std::vector<uint32_t> cb(32 * kCbStride, 0);
+  for (int x = 0; x < 32; ++x)
+    for (int j = 0; j < kCbStride; ++j)
+      cb[x * kCbStride + j] = static_cast<uint32_t>(Binom(x, j));
+  void* d = ctx->Buffer("dp.pascal", cb.size() * 4);
+  TD_HIP(hipMemcpy(d, cb.data(), cb.size() * 4, hipMemcpyHostToDevice));
+  return static_cast<const uint32_t*>(d);
+}
+
+int GridFor(td_ctx* ctx, uint64_t lanes_needed, int block) {
+  const uint64_t blocks = (lanes_needed + block - 1) / block;
+  const uint64_t cap = static_cast<uint64_t>(ctx->prop.multiProcessorCount) *
+                       (2048 / block) * 8;
+  return static_cast<int>(std::max<uint64_t>(1, std::min(blocks, cap)));
+}
+
+template <int LB>
+void LaunchLevels(td_ctx* ctx, const DPGraph& G, const DPGraph* dG,
+                  const uint32_t* cb, uint8_t* arena, uint64_t* launches) {
+  const int R = ctx->dp.tiles_per_lane, block = ctx->dp.block;
+  for (int h = 0; h <= static_cast<int>(G.nh); ++h) {
+    const uint32_t ntiles = static_cast<uint32_t>(Binom(G.nh, h));
+    const int grid = GridFor(ctx, (ntiles + R - 1) / R, block);
+    hipLaunchKernelGGL(DPLevelKernel<LB>, dim3(grid), dim3(block), 0,
+                       ctx->stream, dG, cb, arena, h, ntiles, R);
+    ++*launches;
+  }
+  TD_HIP(hipGetLastError());
+}
+
+void LaunchLevelsLB(td_ctx* ctx, const DPGraph& G, const DPGraph* dG,
+                    const uint32_t* cb, uint8_t* arena, uint64_t* launches) {
+  switch (G.lb) {
+    case 2: LaunchLevels<2>(ctx, G, dG, cb, arena, launches); break;
+    case 3: LaunchLevels<3>(ctx, G, dG, cb, arena, launches); break;
+    case 4: LaunchLevels<4>(ctx, G, dG, cb, arena, launches); break;
+    case 5: LaunchLevels<5>(ctx, G, dG, cb, arena, launches); break;
+    default: throw Error{TD_E_INVALID, "low_bits must be in [2, 5]"};
+  }
+}
+
+// Level sequence of one graph, replayed from a captured hipGraph when the
+// same (n, launch shape, buffers) was seen before.
+void RunLevels(td_ctx* ctx, const DPGraph& G, const DPGraph* dG,
+               const uint32_t* cb, uint8_t* arena, uint64_t* launches) {
+  if (!ctx->dp.use_graph) {
+    LaunchLevelsLB(ctx, G, dG, cb, arena, launches);
+    return;
+  }
+  // key includes the buffer addresses baked into the captured launches
+  td_ctx::GraphKey key{static_cast<int>(G.n), static_cast<int>(G.lb),
+                       ctx->dp.tiles_per_lane, ctx->dp.block};
+  auto& p = ctx->dp_graph_ptrs[key];
+  auto it = ctx->dp_graphs.find(key);
+  if (it != ctx->dp_graphs.end() &&
+      (p.first != static_cast<const void*>(arena) ||
+       p.second != static_cast<const void*>(dG))) {
+    (void)hipGraphExecDestroy(it->second);
+    ctx->dp_graphs.erase(it);
+    it = ctx->dp_graphs.end();
+  }
+  if (it == ctx->dp_graphs.end()) {
+    hipGraph_t graph;
+    uint64_t dummy = 0;
+    TD_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      LaunchLevelsLB(ctx, G, dG, cb, arena, &dummy);
+    } catch (...) {
+      (void)hipStreamEndCapture(ctx->stream, &graph);
+      throw;
+    }
+    TD_HIP(hipStreamEndCapture(ctx->stream, &graph));
+    hipGraphExec_t exec;
+    TD_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    it = ctx->dp_graphs.emplace(key, exec).first;
+    p = {arena, dG};
+  }
+  TD_HIP(hipGraphLaunch(it->second, ctx->stream));
+  ++*launches;
+}
+
+// Replays an elimination order (EliminationTree::Eliminate +
+// Decompose, elimination_tree.cpp:24-70, elimination_tree.hpp:193-215):
+// the parent of v is the vertex whose elimination created v's component.
+int OrderToParent(int n, const uint64_t* adj, const int32_t* order,
+                  int8_t* parent) {
+  struct Comp {
+    uint64_t mask;
+    int creator, depth;
+  };
+  std::vector<Comp> comps;
+  auto split = [&](uint64_t set, int creator, int depth) {
+    while (set) {
+      uint64_t c = set & (0 - set), f = c;
+      while (f) {
+        int u = __builtin_ctzll(f);
+        f &= f - 1;
+        uint64_t nb = adj[u] & set & ~c;
+        c |= nb;
+        f |= nb;
+      }
+      comps.push_back({c, creator, depth});
+      set &= ~c;
+    }
+  };
+  split(n >= 64 ? ~0ull : ((1ull << n) - 1), -1, 0);
+  int td = 0;
+  for (int i = 0; i < n; ++i) {
+    const int v = order[i];
+    TD_REQUIRE(v >= 0 && v < n, TD_E_HIP, "elimination order out of range");
+    size_t k = 0;
+    while (k < comps.size() && !(comps[k].mask >> v & 1)) ++k;
+    TD_REQUIRE(k < comps.size(), TD_E_HIP, "vertex eliminated twice");
+    Comp c = comps[k];
+    comps.erase(comps.begin() + k);
+    parent[v] = static_cast<int8_t>(c.creator);
+    td = std::max(td, c.depth + 1);
+    split(c.mask & ~(1ull << v), v, c.depth + 1);
+  }
+  return td;
+}
+
+template <typename F>
+int Guard(td_ctx* ctx, F&& f) {
+  if (!ctx) return TD_E_INVALID;
+  try {
+    TD_HIP(hipSetDevice(ctx->device));
+    f();
+    ctx->err.clear();
+    return TD_OK;
+  } catch (const Error& e) {
+    ctx->err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return TD_E_INVALID;
+  }
+}
+
+float Elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  TD_HIP(hipEventElapsedTime(&ms, a, b));
+  return ms;
+}
+
+void SolveOne(td_ctx* ctx, int n, const uint64_t* adj, uint8_t* td_out,
+              int8_t* parent_out, td_dp_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  ValidateGraph(n, adj, kMaxDPN);
+  // the N(H) chunk tables cover 28 high vertices
+  const int lb = std::max(ctx->dp.low_bits, n - 28);
+  DPGraph G = MakeGraph(n, adj, lb, 0);
+  const uint64_t tbytes = TableBytes(G);
+  uint8_t* arena = static_cast<uint8_t*>(ctx->Buffer("dp.arena", tbytes));
+  DPGraph* dG = static_cast<DPGraph*>(ctx->Buffer("dp.graph", sizeof(DPGraph)));
+  const uint32_t* cb = PascalDevice(ctx);
+  uint32_t* dstart = static_cast<uint32_t*>(ctx->Buffer("dp.walk_start", 8));
+  int32_t* dsteps = static_cast<int32_t*>(ctx->Buffer("dp.walk_steps", 8));
+  int32_t* dorder = static_cast<int32_t*>(ctx->Buffer("dp.walk_order", 64 * 4));
+  const uint32_t full = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
+  const int32_t steps = n;
+  TD_HIP(hipMemcpyAsync(dG, &G, sizeof(G), hipMemcpyHostToDevice, ctx->stream));
+  TD_HIP(hipMemcpyAsync(dstart, &full, 4, hipMemcpyHostToDevice, ctx->stream));
+  TD_HIP(hipMemcpyAsync(dsteps, &steps, 4, hipMemcpyHostToDevice, ctx->stream));
+  uint64_t launches = 0;
+  TD_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  RunLevels(ctx, G, dG, cb, arena, &launches);
+  TD_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+  hipLaunchKernelGGL(DPWalkKernel, dim3(1), dim3(64), 0, ctx->stream, dG, cb,
+                     arena, dstart, dsteps, dorder, 64);
+  TD_HIP(hipGetLastError());
+  ++launches;
+  TD_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+  int32_t order[64];
+  TD_HIP(hipMemcpyAsync(order, dorder, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  TD_HIP(hipStreamSynchronize(ctx->stream));
+  int8_t parent[64];
+  const int td = OrderToParent(n, adj, order, parent);
+  if (td_out) *td_out = static_cast<uint8_t>(td);
+  if (parent_out) std::memcpy(parent_out, parent, n);
+  ctx->last_n = n;
+  ctx->last_graph = G;
+  ctx->have_dp = true;
+  if (stats) {
+    stats->graphs = 1;
+    stats->subsets = 1ull << n;
+    stats->alg_bytes = (1ull << (n - 1)) * static_cast<uint64_t>(n + 2);
+    stats->launches = launches;
+    stats->level_ms = Elapsed(ctx->ev[0], ctx->ev[1]);
+    stats->tree_ms = Elapsed(ctx->ev[1], ctx->ev[2]);
+    stats->total_ms = std::chrono::duration<double, std::milli>(
+                          std::chrono::steady_clock::now() - t0)
+                          .count();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// batch
+// ---------------------------------------------------------------------------
+template <int LB>
+void LaunchBatch(td_ctx* ctx, int mode, int ngraphs,
+                 const std::vector<DPGraph>& Gs, const DPGraph* dGs,
+                 const uint32_t* cb, uint8_t* arena, uint32_t* dqueue,
+                 uint64_t* launches) {
+  const int R = ctx->dp.tiles_per_lane, block = ctx->dp.block;
+  if (mode == 0) {
+    hipLaunchKernelGGL(ResetCounter, dim3(1), dim3(64), 0, ctx->stream, dqueue);
+    const int grid = ctx->prop.multiProcessorCount * ctx->dp.batch_wg_per_cu;
+    hipLaunchKernelGGL(DPPersistentKernel<LB>, dim3(grid), dim3(256), 0,
+                       ctx->stream, dGs, cb, arena, ngraphs, dqueue, R);
+    *launches += 2;
+  } else if (mode == 1) {
+    int maxnh = 0;
+    for (auto& G : Gs) maxnh = std::max(maxnh, static_cast<int>(G.nh));
+    const uint32_t chunk_tiles = static_cast<uint32_t>(block) * R * 2;
+    std::vector<std::vector<Chunk>> per_level(maxnh + 1);
+    size_t total = 0;
+    for (int h = 0; h <= maxnh; ++h) {
+      for (int i = 0; i < ngraphs; ++i) {
+        if (h > static_cast<int>(Gs[i].nh)) continue;
+        const uint32_t nt = static_cast<uint32_t>(Binom(Gs[i].nh, h));
+        for (uint32_t f = 0; f < nt; f += chunk_tiles)
+          per_level[h].push_back(
+              {static_cast<uint32_t>(i), f, std::min(nt, f + chunk_tiles), 0});
+      }
+      total += per_level[h].size();
+    }
+    Chunk* dch = static_cast<Chunk*>(
+        ctx->Buffer("dp.batch_chunks", std::max<size_t>(1, total) * sizeof(Chunk)));
+    std::vector<Chunk> flat;
+    flat.reserve(total);
+    for (auto& v : per_level) flat.insert(flat.end(), v.begin(), v.end());
+    TD_HIP(hipMemcpyAsync(dch, flat.data(), flat.size() * sizeof(Chunk),
+                          hipMemcpyHostToDevice, ctx->stream));
+    size_t off = 0;
+    for (int h = 0; h <= maxnh; ++h) {
+      const size_t cnt = per_level[h].size();
+      if (cnt) {
+        hipLaunchKernelGGL(DPChunkKernel<LB>, dim3(cnt), dim3(block), 0,
+                           ctx->stream, dGs, cb, arena, dch + off, h, R);
+        ++*launches;
+      }
+      off += cnt;
+    }
+  } else {
+    for (int i = 0; i < ngraphs; ++i)
+      for (int h = 0; h <= static_cast<int>(Gs[i].nh); ++h) {
+        const uint32_t ntiles = static_cast<uint32_t>(Binom(Gs[i].nh, h));
+        const int grid = GridFor(ctx, (ntiles + R - 1) / R, block);
+        hipLaunchKernelGGL(DPLevelKernel<LB>, dim3(grid), dim3(block), 0,
+                           ctx->stream, dGs + i, cb, arena, h, ntiles, R);
+        ++*launches;
+      }
+  }
+  TD_HIP(hipGetLastError());
+}
+
+void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
+                const uint64_t* adj_concat, uint8_t* td_out,
+                int8_t* parent_concat, int mode, td_dp_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  TD_REQUIRE(ngraphs >= 1 && ns && adj_concat, TD_E_INVALID,
+             "empty batch or null pointer");
+  TD_REQUIRE(mode >= 0 && mode <= 2, TD_E_INVALID, "mode must be 0, 1 or 2");
+  const int lb = ctx->dp.low_bits;
+  std::vector<size_t> row_off(ngraphs + 1, 0);
+  for (int i = 0; i < ngraphs; ++i) {
+    TD_REQUIRE(ns[i] >= 1 && ns[i] <= kMaxBatchN, TD_E_INVALID,
+               "batch graphs need n in [1, 26]");
+    row_off[i + 1] = row_off[i] + ns[i];
+  }
+  for (int i = 0; i < ngraphs; ++i)
+    ValidateGraph(ns[i], adj_concat + row_off[i], kMaxBatchN);
+  // largest graphs first so the persistent queue ends with short work
+  std::vector<int> perm(ngraphs);
+  for (int i = 0; i < ngraphs; ++i) perm[i] = i;
+  std::stable_sort(perm.begin(), perm.end(),
+                   [&](int a, int b) { return ns[a] > ns[b]; });
+  std::vector<DPGraph> Gs(ngraphs);
+  uint64_t off = 0;
+  for (int j = 0; j < ngraphs; ++j) {
+    const int i = perm[j];
+    Gs[j] = MakeGraph(ns[i], adj_concat + row_off[i], lb, off);
+    off += (TableBytes(Gs[j]) + 255) / 256 * 256;
+  }
+  uint8_t* arena = static_cast<uint8_t*>(ctx->Buffer("dp.batch_arena", off));
+  DPGraph* dGs = static_cast<DPGraph*>(
+      ctx->Buffer("dp.batch_graphs", sizeof(DPGraph) * ngraphs));
+  const uint32_t* cb = PascalDevice(ctx);
+  uint32_t* dq = static_cast<uint32_t*>(ctx->Buffer("dp.batch_queue", 256));
+  uint32_t* dstart = static_cast<uint32_t*>(ctx->Buffer("dp.batch_start", 4 * ngraphs));
+  int32_t* dsteps = static_cast<int32_t*>(ctx->Buffer("dp.batch_steps", 4 * ngraphs));
+  int32_t* dorder = static_cast<int32_t*>(ctx->Buffer("dp.batch_order", 4 * 32 * ngraphs));
+  std::vector<uint32_t> start(ngraphs);
+  std::vector<int32_t> steps(ngraphs);
+  for (int j = 0; j < ngraphs; ++j) {
+    start[j] = Gs[j].n >= 32 ? 0xFFFFFFFFu : ((1u << Gs[j].n) - 1u);
+    steps[j] = static_cast<int32_t>(Gs[j].n);
+  }
+  TD_HIP(hipMemcpyAsync(dGs, Gs.data(), sizeof(DPGraph) * ngraphs,
+                        hipMemcpyHostToDevice, ctx->stream));
+  TD_HIP(hipMemcpyAsync(dstart, start.data(), 4 * ngraphs, hipMemcpyHostToDevice, ctx->stream));
+  TD_HIP(hipMemcpyAsync(dsteps, steps.data(), 4 * ngraphs, hipMemcpyHostToDevice, ctx->stream));
+  uint64_t launches = 0;
+  TD_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  switch (lb) {
+    case 2: LaunchBatch<2>(ctx, mode, ngraphs, Gs, dGs, cb, arena, dq, &launches); break;
+    case 3: LaunchBatch<3>(ctx, mode, ngraphs, Gs, dGs, cb, arena, dq, &launches); break;
+    case 4: LaunchBatch<4>(ctx, mode, ngraphs, Gs, dGs, cb, arena, dq, &launches); break;
+    case 5: LaunchBatch<5>(ctx, mode, ngraphs, Gs, dGs, cb, arena, dq, &launches); break;
+    default: throw Error{TD_E_INVALID, "low_bits must be in [2, 5]"};
+  }
+  TD_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+  hipLaunchKernelGGL(DPWalkKernel, dim3(ngraphs), dim3(64), 0, ctx->stream,
+                     dGs, cb, arena, dstart, dsteps, dorder, 32);
+  TD_HIP(hipGetLastError());
+  ++launches;
+  TD_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+  std::vector<int32_t> order(static_cast<size_t>(32) * ngraphs);
+  TD_HIP(hipMemcpyAsync(order.data(), dorder, order.size() * 4,
+                        hipMemcpyDeviceToHost, ctx->stream));
+  TD_HIP(hipStreamSynchronize(ctx->stream));
+  uint64_t subsets = 0, alg = 0;
+  for (int j = 0; j < ngraphs; ++j) {
+    const int i = perm[j];
+    const int n = ns[i];
+    int8_t parent[32];
+    const int td = OrderToParent(n, adj_concat + row_off[i],
+                                 order.data() + 32 * j, parent);
+    if (td_out) td_out[i] = static_cast<uint8_t>(td);
+    if (parent_concat) std::memcpy(parent_concat + row_off[i], parent, n);
+    subsets += 1ull << n;
+    alg += (1ull << (n - 1)) * static_cast<uint64_t>(n + 2);
+  }
+  ctx->have_dp = false;  // queries refer to single-graph solves only
+  if (stats) {
+    stats->graphs = static_cast<uint64_t>(ngraphs);
+    stats->subsets = subsets;
+    stats->alg_bytes = alg;
+    stats->launches = launches;
+    stats->level_ms = Elapsed(ctx->ev[0], ctx->ev[1]);
+    stats->tree_ms = Elapsed(ctx->ev[1], ctx->ev[2]);
+    stats->total_ms = std::chrono::duration<double, std::milli>(
+                          std::chrono::steady_clock::now() - t0)
+                          .count();
+  }
+}
+
+}  // namespace
+}  // namespace tdhip
+
+using namespace tdhip;  // NOLINT
+
+extern "C" {
+
+int td_dp_solve(td_ctx* ctx, int32_t n, const uint64_t* adj,
+                uint8_t* treedepth_out, int8_t* parent_out,
+                td_dp_stats* stats) {
+  return Guard(ctx, [&] {
+    SolveOne(ctx, n, adj, treedepth_out, parent_out, stats);
+  });
+}
+
+int td_dp_get_treedepth(td_ctx* ctx, int32_t k, uint64_t code,
+                        uint32_t* td_out) {
+  return Guard(ctx, [&] {
+    TD_REQUIRE(ctx->have_dp, TD_E_STATE, "no td_dp_solve result to query");
+    TD_REQUIRE(td_out != nullptr, TD_E_INVALID, "td_out is null");
+    const int n = ctx->last_n;
+    TD_REQUIRE(k >= 0 && k <= n && code < Binom(n, k), TD_E_STATE,
+               "subset code out of range");
+    const uint64_t S = UnrankColex(n, k, code);
+    const DPGraph& G = ctx->last_graph;
+    const uint64_t L = S & ((1ull << G.lb) - 1);
+    const uint64_t H = S >> G.lb;
+    const int h = __builtin_popcountll(H);
+    const uint64_t addr =
+        ((static_cast<uint64_t>(G.lvl[h]) + RankColex(H)) << G.lb) + L;
+    uint8_t raw = 0;
+    const uint8_t* arena = static_cast<const uint8_t*>(ctx->bufs["dp.arena"].p);
+    TD_HIP(hipMemcpy(&raw, arena + addr, 1, hipMemcpyDeviceToHost));
+    *td_out = raw >> 1;
+  });
+}
+
+int td_dp_get_level(td_ctx* ctx, int32_t k, uint8_t* out, uint64_t out_len) {
+  return Guard(ctx, [&] {
+    TD_REQUIRE(ctx->have_dp, TD_E_STATE, "no td_dp_solve result to query");
+    const int n = ctx->last_n;
+    TD_REQUIRE(k >= 0 && k <= n, TD_E_STATE, "level out of range");
+    const uint64_t count = Binom(n, k);
+    TD_REQUIRE(out && out_len >= count, TD_E_INVALID, "output too small");
+    uint8_t* d = static_cast<uint8_t*>(ctx->Buffer("dp.export", count));
+    const DPGraph* dG = static_cast<const DPGraph*>(ctx->bufs["dp.graph"].p);
+    const uint8_t* arena = static_cast<const uint8_t*>(ctx->bufs["dp.arena"].p);
+    const int grid = GridFor(ctx, count, 256);
+    hipLaunchKernelGGL(DPExportLevelKernel, dim3(grid), dim3(256), 0,
+                       ctx->stream, dG, PascalDevice(ctx), arena, k,
+                       static_cast<uint32_t>(count), d);
+    TD_HIP(hipGetLastError());
+    TD_HIP(hipMemcpyAsync(out, d, count, hipMemcpyDeviceToHost, ctx->stream));
+    TD_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int td_dp_get_elimination(td_ctx* ctx, int32_t nverts, int32_t subset_size,
+                          uint64_t subset_code, int32_t* order_out) {
+  return Guard(ctx, [&] {
+    TD_REQUIRE(ctx->have_dp, TD_E_STATE, "no td_dp_solve result to query");
+    const int n = ctx->last_n;
+    TD_REQUIRE(nverts == n, TD_E_INVALID, "nverts differs from the solved graph");
+    TD_REQUIRE(subset_size >= 0 && subset_size <= n &&
+                   subset_code < Binom(n, subset_size),
+               TD_E_STATE, "subset code out of range");
+    TD_REQUIRE(order_out != nullptr || subset_size == 0, TD_E_INVALID,
+               "order_out is null");
+    if (subset_size == 0) return;
+    const uint32_t S = static_cast<uint32_t>(UnrankColex(n, subset_size, subset_code));
+    const int32_t steps = subset_size;
+    uint32_t* dstart = static_cast<uint32_t*>(ctx->Buffer("dp.walk_start", 8));
+    int32_t* dsteps = static_cast<int32_t*>(ctx->Buffer("dp.walk_steps", 8));
+    int32_t* dorder = static_cast<int32_t*>(ctx->Buffer("dp.walk_order", 64 * 4));
+    TD_HIP(hipMemcpyAsync(dstart, &S, 4, hipMemcpyHostToDevice, ctx->stream));
+    TD_HIP(hipMemcpyAsync(dsteps, &steps, 4, hipMemcpyHostToDevice, ctx->stream));
+    const DPGraph* dG = static_cast<const DPGraph*>(ctx->bufs["dp.graph"].p);
+    const uint8_t* arena = static_cast<const uint8_t*>(ctx->bufs["dp.arena"].p);
+    hipLaunchKernelGGL(DPWalkKernel, dim3(1), dim3(64), 0, ctx->stream, dG,
+                       PascalDevice(ctx), arena, dstart, dsteps, dorder, 64);
+    TD_HIP(hipGetLastError());
+    TD_HIP(hipMemcpyAsync(order_out, dorder, 4 * subset_size,
+                          hipMemcpyDeviceToHost, ctx->stream));
+    TD_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int td_dp_solve_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
+                      const uint64_t* adj_concat, uint8_t* treedepth_out,
+                      int8_t* parent_concat, int32_t mode,
+                      td_dp_stats* stats) {
+  return Guard(ctx, [&] {
+    SolveBatch(ctx, ngraphs, n, adj_concat, treedepth_out, parent_concat, mode,
+               stats);
+  });
+}
+
+int td_dp_autotune(td_ctx* ctx, int32_t n, const char* cache_path,
+                   double* best_ms_out) {
+  return Guard(ctx, [&] {
+    TD_REQUIRE(n >= 8 && n <= kMaxDPN, TD_E_INVALID, "autotune n in [8, 32]");
+    // seeded G(n, 0.25) (SplitMix64)
+    std::vector<uint64_t> adj(n, 0);
+    uint64_t s = 0x9E3779B97F4A7C15ull * (n + 1);
+    auto next = [&]() {
+      s += 0x9E3779B97F4A7C15ull;
+      uint64_t z = s;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      return z ^ (z >> 31);
+    };
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j)
+        if ((next() >> 11) * (1.0 / (1ull << 53)) < 0.25) {
+          adj[i] |= 1ull << j;
+          adj[j] |= 1ull << i;
+        }
+    const DPConfig saved = ctx->dp;
+    DPConfig best = saved;
+    double best_ms = 1e30;
+    std::string log;
+    const int lbs[] = {3, 4, 5};
+    const int rs[] = {1, 2, 4, 8};
+    const int blocks[] = {128, 256, 512};
+    for (int lb : lbs)
+      for (int r : rs)
+        for (int b : blocks) {
+          ctx->dp = saved;
+          ctx->dp.low_bits = lb;
+          ctx->dp.tiles_per_lane = r;
+          ctx->dp.block = b;
+          ctx->dp.use_graph = 0;
+          td_dp_stats st{};
+          uint8_t td;
+          int8_t par[64];
+          SolveOne(ctx, n, adj.data(), &td, par, &st);  // warm-up
+          double t = 0;
+          for (int rep = 0; rep < 3; ++rep) {
+            SolveOne(ctx, n, adj.data(), &td, par, &st);
+            t += st.level_ms;
+          }
+          t /= 3;
+          char line[160];
+          std::snprintf(line, sizeof(line),
+                        "{\"n\":%d,\"low_bits\":%d,\"tiles_per_lane\":%d,"
+                        "\"block\":%d,\"level_ms\":%.4f}\n",
+                        n, lb, r, b, t);
+          log += line;
+          if (t < best_ms) {
+            best_ms = t;
+            best = ctx->dp;
+          }
+        }
+    best.use_graph = saved.use_graph;
+    ctx->dp = best;
+    for (auto& kv : ctx->dp_graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->dp_graphs.clear();
+    if (cache_path) {
+      std::ofstream f(cache_path, std::ios::app);
+      f << log;
+      char line[200];
+      std::snprintf(line, sizeof(line),
+                    "{\"n\":%d,\"best\":true,\"low_bits\":%d,"
+                    "\"tiles_per_lane\":%d,\"block\":%d,\"level_ms\":%.4f,"
+                    "\"arch\":\"%s\"}\n",
+                    n, best.low_bits, best.tiles_per_lane, best.block, best_ms,
+                    ctx->prop.gcnArchName);
+      f << line;
+    }
+    if (best_ms_out) *best_ms_out = best_ms;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+/*
+ * tdhip.h -- C ABI of the MI355X exact treedepth library (libtdhip.so).
+ *
+ * The reference (Zackere/GiSBDW) exposes its GPU paths as C++ classes over
+ * Boost graphs; this header is the flat boundary those classes are rebuilt
+ * on.  Each entry point names the reference interface it replaces.
+ *
+ * Graph input everywhere: n vertices 0..n-1, adjacency as n uint64 bitmask
+ * rows (bit u of row v set <=> edge {u,v}); rows must be symmetric with no
+ * self loops and no bits >= n.  The caller owns every host buffer; the
+ * context owns every device buffer.  Calls are synchronous.  One context per
+ * host thread; the library holds no global mutable state across contexts.
+ * No C++ exception crosses this boundary: every call returns a td_status.
+ */
+#ifndef TDHIP_H_
+#define TDHIP_H_
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum td_status {
+  TD_OK = 0,
+  TD_E_INVALID = 1, /* bad argument or graph (self loop, asymmetric, n)   */
+  TD_E_NOMEM = 2,   /* the context's byte budget or the device is full    */
+  TD_E_HIP = 3,     /* a HIP runtime call failed                          */
+  TD_E_BUDGET = 4,  /* search stopped by its node/time budget (uncertified) */
+  TD_E_STATE = 5    /* query before any solve, or out-of-range query      */
+} td_status;
+
+typedef struct td_ctx td_ctx;
+
+/* Context: owns a HIP stream, a device arena and the tuning parameters.
+ * mem_budget_bytes = 0 means "whatever the device has".
+ * Replaces the implicit device-0 + Thrust allocations of
+ * Treedepth/src/dynamic_gpu/dynamic_gpu.cu:171-201 and
+ * Treedepth/src/bnb_gpu/bnb_gpu.cu:264-285. */
+int td_ctx_create(td_ctx** out, int device, uint64_t mem_budget_bytes);
+void td_ctx_destroy(td_ctx* ctx);
+/* Last error message of this context ("" if none). */
+const char* td_last_error(const td_ctx* ctx);
+/* Tuning knobs (see td_dp_autotune): "dp.low_bits" (3..5), "dp.tiles_per_lane"
+ * (1..16), "dp.block" (64..1024), "dp.batch_wg_per_cu", "dp.use_graph" (0/1),
+ * "bnb.waves", "bnb.node_budget", "bnb.time_budget_ms", "bnb.memo_log2". */
+int td_ctx_set_param(td_ctx* ctx, const char* key, int64_t value);
+int td_ctx_get_param(const td_ctx* ctx, const char* key, int64_t* value);
+
+/* ---------------------------------------------------------------------- */
+/* Subset DP                                                               */
+/* ---------------------------------------------------------------------- */
+typedef struct td_dp_stats {
+  uint64_t graphs;       /* graphs solved by the call                    */
+  uint64_t subsets;      /* sum over graphs of 2^n                        */
+  uint64_t alg_bytes;    /* sum over graphs of 2^(n-1) (n+2)              */
+  uint64_t launches;     /* kernel launches (or graph replays)            */
+  double level_ms;       /* GPU time of the level kernels (HIP events)    */
+  double tree_ms;        /* GPU time of the elimination-order kernel      */
+  double total_ms;       /* host wall time of the whole call              */
+} td_dp_stats;
+
+/* DynamicGPU::operator()(g) + GetElimination(n, n, 0) + EliminationTree +
+ * Decompose (Treedepth/src/dynamic_gpu/dynamic_gpu.hpp:38-64,
+ * Treedepth/main.cpp:187-195).  n <= 32.  Writes the treedepth and the
+ * canonical decomposition as a parent array (-1 = root).  A disconnected
+ * input yields a forest: one canonical tree per component, roots -1, and
+ * treedepth = the largest tree's depth (the reference rejects such input).
+ * The level tables stay on the device for the queries below. stats nullable. */
+int td_dp_solve(td_ctx* ctx, int32_t n, const uint64_t* adj,
+                uint8_t* treedepth_out, int8_t* parent_out,
+                td_dp_stats* stats);
+
+/* DynamicCPU::GetTreedepth(nverts, subset_size, subset_code)
+ * (Treedepth/src/dynamic_cpu/dynamic_cpu.hpp:34-36, .cpp:29-39) over the
+ * last td_dp_solve: treedepth of the subset with this colex code.  Unlike
+ * the reference, disconnected subsets are answered too. */
+int td_dp_get_treedepth(td_ctx* ctx, int32_t subset_size,
+                        uint64_t subset_code, uint32_t* td_out);
+
+/* The level-k table of the last td_dp_solve in colex order:
+ * out[code] = td of the k-subset with that code, out_len >= C(n, k).
+ * (The reference keeps these as history_, dynamic_gpu.hpp:33-35.) */
+int td_dp_get_level(td_ctx* ctx, int32_t k, uint8_t* out, uint64_t out_len);
+
+/* DynamicGPU::GetElimination<V>(nverts, subset_size, subset_code)
+ * (Treedepth/src/dynamic_gpu/dynamic_gpu.hpp:49-64): the elimination order
+ * of that subset, subset_size vertices written to order_out. */
+int td_dp_get_elimination(td_ctx* ctx, int32_t nverts, int32_t subset_size,
+                          uint64_t subset_code, int32_t* order_out);
+
+/* Many graphs in one call (no reference counterpart: the reference loops
+ * over input files one at a time, Treedepth/main.cpp:130).  n[i] <= 26;
+ * adj_concat holds the rows of graph 0, then graph 1, ...; parent_concat
+ * likewise (sum n[i] entries).  mode: 0 = one persistent launch (graphs
+ * pulled from a device queue, all levels inside), 1 = level-synchronous
+ * launches over the whole batch, 2 = one launch per graph and level (the
+ * baseline the persistent launch is measured against). */
+int td_dp_solve_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
+                      const uint64_t* adj_concat, uint8_t* treedepth_out,
+                      int8_t* parent_concat, int32_t mode,
+                      td_dp_stats* stats);
+
+/* Autotuner: times the DP level kernel for every candidate (low_bits,
+ * tiles_per_lane, block) on a seeded G(n, 0.25) graph and keeps the best in
+ * the context (and in cache_path as JSON lines when non-null).  best_ms_out
+ * nullable. */
+int td_dp_autotune(td_ctx* ctx, int32_t n, const char* cache_path,
+                   double* best_ms_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TDHIP_H_ */

@@ -1,0 +1,111 @@
+/*
+ * oracle.h -- CPU restatement of the GiSBDW treedepth algorithms.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library (gisbdw_amd/)
+ * links, loads or calls this code.  It is used by tests/, by
+ * __graft_entry__.smoke() as the checker, and by bench.py's cpu_baseline leg.
+ *
+ * Graph convention everywhere: n vertices 0..n-1 (1 <= n <= 64), adjacency as
+ * n rows of uint64 bitmasks, symmetric, no self loops.
+ *
+ * Parity status of each function (what pins it) is written next to it.
+ */
+#ifndef GISBDW_ORACLE_H_
+#define GISBDW_ORACLE_H_
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- colex codec (Treedepth/src/set_encoder/set_encoder.{hpp,cu}) ----- */
+/* Pinned: Pascal identity + closed form; bijection; exclude identity;
+ * compared with the reference's own set_encoder compiled under oracle/_ref. */
+uint64_t or_nchoosek(uint64_t n, uint64_t k);
+uint64_t or_encode(const int32_t* sorted_set, int32_t k);
+uint64_t or_encode_exclude(const int32_t* sorted_set, int32_t k, int32_t excl);
+void or_decode(int32_t nverts, int32_t k, uint64_t code, int32_t* out);
+
+/* ---- EliminationTree (Treedepth/src/elimination_tree/) --------------- */
+/* Replays an elimination order.  After each step i (0..n) writes the list of
+ * current components as (vertex mask, depth) pairs, ordered by their lowest
+ * vertex (ComponentCmp).  comp_masks/comp_depths have room for
+ * (n+1)*n entries; comp_counts[i] = number of components after step i.
+ * Then Decompose(): parent[v] (-1 for the root), treedepth, root.
+ * Pinned: the four KAT cases of elimination_tree_unittest.cpp. */
+int or_elimtree_replay(int32_t n, const uint64_t* adj, const int32_t* order,
+                       uint64_t* comp_masks, int32_t* comp_depths,
+                       int32_t* comp_counts, int32_t* parent,
+                       int32_t* treedepth, int32_t* root);
+/* Same replay, then Merge() back step by step; returns 0 if every
+ * intermediate component list equals the one recorded on the way down and
+ * each Merge returns the vertex eliminated at that step. */
+int or_elimtree_merge_check(int32_t n, const uint64_t* adj,
+                            const int32_t* order);
+
+/* ---- DynamicCPU (Treedepth/src/dynamic_cpu/dynamic_cpu.{hpp,cpp}) ----- */
+/* Faithful: std::map per level, connected sets only, GetTDDecomp(0, g).
+ * Returns treedepth (UINT32_MAX for a disconnected input, like the
+ * reference) and the decomposition as a parent array.
+ * Pinned: closed forms, brute force n<=8, agreement with or_dp_fast. */
+uint32_t or_dyncpu(int32_t n, const uint64_t* adj, int32_t* parent,
+                   int32_t* root);
+/* td of one subset from the faithful DynamicCPU tables (GetTreedepth);
+ * UINT64_MAX when the subset is disconnected (not stored). Runs the DP. */
+uint64_t or_dyncpu_subset_td(int32_t n, const uint64_t* adj, int32_t k,
+                             uint64_t code);
+
+/* ---- DynamicCPUImprov (dynamic_cpu_improv.{hpp,cpp}) ------------------ */
+/* Top-down memo over components, bitmask keys. Connected input only. */
+uint32_t or_dyncpu_improv(int32_t n, const uint64_t* adj, int32_t* parent,
+                          int32_t* root);
+
+/* ---- fast flat-table DP (semantics of dynamic_gpu.cu:40-113) ---------- */
+/* uint8 td per subset, one colex-ranked table per level; component-aware
+ * recurrence with components looked up at their own levels.  levels_out
+ * (nullable) receives all levels concatenated (level k at offset
+ * sum_{j<k} C(n,j)), i.e. 2^n bytes.  Canonical tree (forest for a
+ * disconnected input) into parent.  threads<=0 means all cores.
+ * Returns treedepth. n <= 32. */
+int32_t or_dp_fast(int32_t n, const uint64_t* adj, uint8_t* levels_out,
+                   int8_t* parent, int32_t threads);
+
+/* ---- certifier (Treedepth/common/utils/utils.hpp:50-86) --------------- */
+/* 1 if parent[] is a rooted forest on 0..n-1, every edge joins an
+ * ancestor/descendant pair, and the depth (roots at depth 1) equals td. */
+int or_certify(int32_t n, const uint64_t* adj, const int8_t* parent,
+               int32_t td);
+
+/* ---- brute force by definition (n <= 8) -------------------------------- */
+/* Minimum depth over ALL rooted forests on the vertex set whose closure
+ * contains every edge.  Independent of every recurrence above. */
+int32_t or_brute_td(int32_t n, const uint64_t* adj);
+
+/* ---- BranchAndBound (Treedepth/src/branch_and_bound/) ----------------- */
+/* lb_kind: 0 = EdgeLowerBound, 1 = BasicLowerBound.
+ * heur_chain: 0 = none (initial best treedepth = n+1, the trivial path
+ * bound), 1 = HighestDegree(SpanningTree(Variance(nullptr, 1.0,0.2,0.8))),
+ * 2 = HighestDegree only, 3 = SpanningTree only, 4 = Variance only.
+ * node_budget: 0 = unlimited; otherwise stop after that many Algorithm()
+ * calls and return -1 in *certified.  Returns treedepth; parent/root of the
+ * best tree. Connected input only. */
+int32_t or_bnb(int32_t n, const uint64_t* adj, int32_t lb_kind,
+               int32_t heur_chain, uint64_t node_budget, int32_t* parent,
+               int32_t* root, int32_t* certified, uint64_t* nodes);
+/* The heuristics alone (upper bounds): returns treedepth; parent/root. */
+int32_t or_heuristic(int32_t n, const uint64_t* adj, int32_t heur_chain,
+                     int32_t* parent, int32_t* root);
+/* EdgeLowerBound as the reference computes it (double sqrt/ceil). */
+uint32_t or_edge_lb_double(int32_t n, int32_t e);
+
+/* ---- exact memoised solver (checker for n in [40,64]) ------------------ */
+/* Top-down decision search with a component memo: exact treedepth for
+ * graphs too large for the subset DP.  Independent code from the GPU B&B.
+ * Returns -1 when the node budget (0 = unlimited) is exhausted. */
+int32_t or_exact_memo(int32_t n, const uint64_t* adj, uint64_t node_budget,
+                      int8_t* parent, uint64_t* nodes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif  // GISBDW_ORACLE_H_

@@ -1,0 +1,390 @@
+// oracle_bnb.cpp -- CPU restatement of the GiSBDW BranchAndBound, its lower
+// bound plugins and its upper-bound heuristics, plus an exact memoised
+// decision search used as the checker for graphs too large for the subset DP.
+//
+// TEST INFRASTRUCTURE ONLY: used by tests/ and bench.py's cpu_baseline leg.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <numeric>
+#include <set>
+#include <unordered_map>
+#include <vector>
+
+#include "oracle.h"
+#include "oracle_internal.hpp"
+
+namespace oracle {
+
+// EdgeLowerBound::Get (edge_lower_bound.hpp:13-20): double arithmetic on
+// int n, e exactly as written there.
+unsigned EdgeLB(int n, int e) {
+  return static_cast<unsigned>(
+      std::ceil(0.5 + n - std::sqrt(0.25 + n * n - n - 2 * e)));
+}
+
+struct Res {
+  std::vector<int> parent;
+  int treedepth;
+  int root;
+};
+
+static Res FromTree(const ElimTree& et) {
+  auto r = et.Decompose();
+  return Res{r.parent, r.treedepth, r.root};
+}
+
+// HighestDegreeHeuristic::Get (highest_degree_heuristic.hpp:19-45)
+static Res HighestDegree(int n, const uint64_t* adj) {
+  ElimTree t(n, adj);
+  while (t.HasComponents()) {
+    uint64_t c = t.FirstComponent().mask;
+    int best_v = -1;
+    int best_deg = -1;
+    for (uint64_t m = c; m; m &= m - 1) {
+      int v = __builtin_ctzll(m);
+      int d = __builtin_popcountll(adj[v] & c);
+      if (best_v < 0 || d > best_deg) {  // first strictly greater wins
+        best_v = v;
+        best_deg = d;
+      }
+    }
+    t.Eliminate(best_v);
+  }
+  return FromTree(t);
+}
+
+// VarianceHeuristic::Get (variance_heuristic.hpp:24-78), A/D/E as float.
+static Res Variance(int n, const uint64_t* adj, float A, float D, float E) {
+  ElimTree t(n, adj);
+  while (t.HasComponents()) {
+    double best_coef = -1;
+    int best_deg = -1;
+    int best_v = 0;
+    uint64_t c = t.FirstComponent().mask;
+    for (uint64_t m = c; m; m &= m - 1) {
+      int v = __builtin_ctzll(m);
+      int deg = __builtin_popcountll(adj[v] & c);
+      auto comps = t.Eliminate(v);
+      double avg = 0.0;
+      for (uint64_t cc : comps) avg += __builtin_popcountll(cc);
+      avg = avg / comps.size();
+      double gamma = 0.0;
+      for (uint64_t cc : comps) {
+        double d = __builtin_popcountll(cc) - avg;
+        gamma += d * d;
+      }
+      double coef = D / (std::sqrt(gamma + A)) + E * comps.size();
+      t.Merge();
+      if (coef > best_coef || (coef == best_coef && deg > best_deg)) {
+        best_coef = coef;
+        best_deg = deg;
+        best_v = v;
+      }
+    }
+    t.Eliminate(best_v);
+  }
+  return FromTree(t);
+}
+
+// SpanningTree (graph_gen.hpp:47-62): union-find over edges in (i asc,
+// target asc) order; keep the edges that join two sets.
+static std::vector<uint64_t> SpanningTreeOf(int n, const uint64_t* adj) {
+  std::vector<int> uf(n);
+  std::iota(uf.begin(), uf.end(), 0);
+  auto find = [&](int x) {
+    while (uf[x] != x) x = uf[x] = uf[uf[x]];
+    return x;
+  };
+  std::vector<uint64_t> t(n, 0);
+  for (int i = 0; i < n; ++i)
+    for (uint64_t m = adj[i]; m; m &= m - 1) {
+      int j = __builtin_ctzll(m);
+      int a = find(i), b = find(j);
+      if (a != b) {
+        uf[b] = a;
+        t[i] |= 1ull << j;
+        t[j] |= 1ull << i;
+      }
+    }
+  return t;
+}
+
+// SpanningTreeHeuristic::Get (spanning_tree_heuristic.hpp:20-83)
+static Res SpanningTreeH(int n, const uint64_t* adj) {
+  auto st = SpanningTreeOf(n, adj);
+  ElimTree ste(n, st.data());
+  ElimTree res(n, adj);
+  while (ste.HasComponents()) {
+    uint64_t c = ste.FirstComponent().mask;
+    int sz = __builtin_popcountll(c);
+    if (sz < 3) {
+      int v = __builtin_ctzll(c);
+      res.Eliminate(v);
+      ste.Eliminate(v);
+      continue;
+    }
+    uint64_t taken = 0;
+    for (uint64_t m = c; m; m &= m - 1) {
+      int v = __builtin_ctzll(m);
+      if (__builtin_popcountll(st[v] & c) == 1) taken |= 1ull << v;
+    }
+    while (__builtin_popcountll(taken) + 2 < sz) {
+      uint64_t now = 0;
+      for (uint64_t m = c & ~taken; m; m &= m - 1) {
+        int v = __builtin_ctzll(m);
+        if (__builtin_popcountll(st[v] & c & ~taken) == 1) now |= 1ull << v;
+      }
+      taken |= now;
+    }
+    int v = __builtin_ctzll(c & ~taken);
+    res.Eliminate(v);
+    ste.Eliminate(v);
+  }
+  return FromTree(res);
+}
+
+static Res Heuristic(int n, const uint64_t* adj, int chain) {
+  // Decorators return the inner result only when strictly better.
+  switch (chain) {
+    case 2:
+      return HighestDegree(n, adj);
+    case 3:
+      return SpanningTreeH(n, adj);
+    case 4:
+      return Variance(n, adj, 1.0f, 0.2f, 0.8f);
+    case 1: {
+      Res var = Variance(n, adj, 1.0f, 0.2f, 0.8f);
+      Res st = SpanningTreeH(n, adj);
+      if (var.treedepth < st.treedepth) st = var;
+      Res hd = HighestDegree(n, adj);
+      if (st.treedepth < hd.treedepth) hd = st;
+      return hd;
+    }
+    default: {
+      Res r;
+      r.parent.assign(n, -1);
+      r.treedepth = n + 1;
+      r.root = -1;
+      return r;
+    }
+  }
+}
+
+// BranchAndBound::Algorithm (branch_and_bound.cpp:51-115) with a
+// LowerBoundInfo-only plugin (Edge or Basic); attempt order = ascending.
+struct BnB {
+  ElimTree* et;
+  int lb_kind;
+  Res best;
+  uint64_t nodes = 0, budget = 0;
+  bool aborted = false;
+
+  unsigned LB(const ElimTree::Comp& c) const {
+    int nv = __builtin_popcountll(c.mask);
+    if (lb_kind == 1) return nv > 1 ? 2 : 1;  // basic_lower_bound.hpp:11-17
+    return EdgeLB(nv, EdgesIn(et->Adj(), c.mask));
+  }
+
+  void Algorithm() {
+    if (aborted) return;
+    if (budget && ++nodes > budget) {
+      aborted = true;
+      return;
+    }
+    if (!budget) ++nodes;
+    if (!et->HasComponents()) {
+      auto r = et->Decompose();
+      if (r.treedepth < best.treedepth) best = Res{r.parent, r.treedepth, r.root};
+      return;
+    }
+    auto comps = et->AllComponents();
+    const auto& first = comps[0];
+    unsigned first_lb = LB(first);
+    for (size_t i = 1; i < comps.size(); ++i)
+      if (comps[i].depth + LB(comps[i]) >= (unsigned)best.treedepth) return;
+    if (first.depth + first_lb >= (unsigned)best.treedepth) return;
+    uint64_t mask = first.mask;
+    for (uint64_t m = mask; m; m &= m - 1) {
+      et->Eliminate(__builtin_ctzll(m));
+      Algorithm();
+      et->Merge();
+      if (aborted) return;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Exact memoised decision search: feasible(C, k) <=> td(C) <= k for a
+// connected vertex set C.  Prunes only with proven bounds: |C| <= k,
+// EdgeLowerBound, degeneracy + 1 (td - 1 >= treewidth >= degeneracy) and
+// memoised proven lower bounds.
+// ---------------------------------------------------------------------------
+struct Exact {
+  int n;
+  const uint64_t* adj;
+  std::unordered_map<uint64_t, uint8_t> lb;                 // td(C) >= lb
+  std::unordered_map<uint64_t, std::pair<uint8_t, int8_t>> sol;  // k, root
+  uint64_t nodes = 0, budget = 0;
+  bool aborted = false;
+
+  int Degeneracy(uint64_t c) const {
+    int best = 0;
+    while (c) {
+      int mind = 99, mv = -1;
+      for (uint64_t m = c; m; m &= m - 1) {
+        int v = __builtin_ctzll(m);
+        int d = __builtin_popcountll(adj[v] & c);
+        if (d < mind) {
+          mind = d;
+          mv = v;
+        }
+      }
+      best = std::max(best, mind);
+      c &= ~(1ull << mv);
+    }
+    return best;
+  }
+
+  int StaticLB(uint64_t c) {
+    int nv = __builtin_popcountll(c);
+    int e = EdgesIn(adj, c);
+    int a = (int)EdgeLB(nv, e);
+    int b = Degeneracy(c) + 1;
+    return std::max(a, b);
+  }
+
+  bool Feasible(uint64_t c, int k) {
+    int nv = __builtin_popcountll(c);
+    if (k >= nv) {
+      // path elimination: td(C) <= |C|; record a trivial solution lazily
+      if (!sol.count(c) || sol[c].first > nv) sol[c] = {(uint8_t)nv, -1};
+      return true;
+    }
+    if (k <= 0) return false;
+    auto s = sol.find(c);
+    if (s != sol.end() && s->second.first <= k) return true;
+    auto l = lb.find(c);
+    int known = l == lb.end() ? 0 : l->second;
+    if (known == 0) {
+      known = StaticLB(c);
+      lb[c] = (uint8_t)known;
+    }
+    if (known > k) return false;
+    if (aborted) return false;
+    if (budget && ++nodes > budget) {
+      aborted = true;
+      return false;
+    }
+    if (!budget) ++nodes;
+    // try roots by descending degree inside C, then ascending id
+    int order[64], cnt = 0;
+    for (uint64_t m = c; m; m &= m - 1) order[cnt++] = __builtin_ctzll(m);
+    std::stable_sort(order, order + cnt, [&](int a, int b) {
+      return __builtin_popcountll(adj[a] & c) >
+             __builtin_popcountll(adj[b] & c);
+    });
+    for (int i = 0; i < cnt; ++i) {
+      int v = order[i];
+      auto comps = Components(adj, c & ~(1ull << v));
+      std::sort(comps.begin(), comps.end(), [](uint64_t a, uint64_t b) {
+        return __builtin_popcountll(a) > __builtin_popcountll(b);
+      });
+      bool ok = true;
+      for (uint64_t cc : comps)
+        if (!Feasible(cc, k - 1)) {
+          ok = false;
+          break;
+        }
+      if (aborted) return false;
+      if (ok) {
+        sol[c] = {(uint8_t)k, (int8_t)v};
+        return true;
+      }
+    }
+    lb[c] = (uint8_t)(k + 1);
+    return false;
+  }
+
+  int Solve(uint64_t c) {
+    int k = StaticLB(c);
+    while (!Feasible(c, k)) {
+      if (aborted) return -1;
+      ++k;
+    }
+    return k;
+  }
+
+  void Tree(uint64_t c, int k, int par, int8_t* parent) {
+    // rebuild from stored roots; a trivial (-1) entry means "eliminate as a
+    // path": lowest vertex first
+    auto s = sol.find(c);
+    int root;
+    int kk;
+    if (s == sol.end() || s->second.second < 0) {
+      root = __builtin_ctzll(c);
+      kk = __builtin_popcountll(c);
+    } else {
+      root = s->second.second;
+      kk = s->second.first;
+    }
+    (void)k;
+    parent[root] = (int8_t)par;
+    // every child component was proven feasible at kk-1 when c was proven
+    // at kk, so its own stored entry has k <= kk-1
+    for (uint64_t cc : Components(adj, c & ~(1ull << root)))
+      Tree(cc, kk - 1, root, parent);
+  }
+};
+
+}  // namespace oracle
+
+using namespace oracle;  // NOLINT
+
+extern "C" {
+
+uint32_t or_edge_lb_double(int32_t n, int32_t e) { return EdgeLB(n, e); }
+
+int32_t or_heuristic(int32_t n, const uint64_t* adj, int32_t chain,
+                     int32_t* parent, int32_t* root) {
+  Res r = Heuristic(n, adj, chain);
+  for (int v = 0; v < n; ++v) parent[v] = r.parent[v];
+  *root = r.root;
+  return r.treedepth;
+}
+
+int32_t or_bnb(int32_t n, const uint64_t* adj, int32_t lb_kind,
+               int32_t heur_chain, uint64_t node_budget, int32_t* parent,
+               int32_t* root, int32_t* certified, uint64_t* nodes) {
+  if (Components(adj, FullMask(n)).size() != 1) return -1;
+  ElimTree et(n, adj);
+  BnB b{&et, lb_kind, Heuristic(n, adj, heur_chain)};
+  b.budget = node_budget;
+  b.Algorithm();
+  for (int v = 0; v < n; ++v) parent[v] = b.best.parent[v];
+  *root = b.best.root;
+  *certified = b.aborted ? 0 : 1;
+  if (nodes) *nodes = b.nodes;
+  return b.best.treedepth;
+}
+
+int32_t or_exact_memo(int32_t n, const uint64_t* adj, uint64_t node_budget,
+                      int8_t* parent, uint64_t* nodes) {
+  Exact ex{n, adj};
+  ex.budget = node_budget;
+  int best = 0;
+  for (uint64_t c : Components(adj, FullMask(n))) {
+    int t = ex.Solve(c);
+    if (t < 0) {
+      if (nodes) *nodes = ex.nodes;
+      return -1;
+    }
+    best = std::max(best, t);
+    if (parent) ex.Tree(c, t, -1, parent);
+  }
+  if (nodes) *nodes = ex.nodes;
+  return best;
+}
+
+}  // extern "C"

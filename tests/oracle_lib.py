@@ -1,0 +1,175 @@
+"""ctypes access to the CPU restatement (oracle/liboracle.so) for the tests.
+
+The oracle is the checker only: product code never imports this module.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libref_set_encoder.so"
+
+P = C.POINTER
+
+
+def _p(a, t):
+    return a.ctypes.data_as(P(t))
+
+
+class Oracle:
+    def __init__(self):
+        if not ORACLE_SO.exists():
+            import subprocess
+
+            subprocess.run(["make", "-s", "-C", str(ORACLE_SO.parent)], check=True)
+        L = C.CDLL(str(ORACLE_SO))
+        sig = {
+            "or_nchoosek": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+            "or_encode": (C.c_uint64, [P(C.c_int32), C.c_int32]),
+            "or_encode_exclude": (C.c_uint64, [P(C.c_int32), C.c_int32, C.c_int32]),
+            "or_decode": (None, [C.c_int32, C.c_int32, C.c_uint64, P(C.c_int32)]),
+            "or_elimtree_replay": (C.c_int, [C.c_int32, P(C.c_uint64), P(C.c_int32),
+                                             P(C.c_uint64), P(C.c_int32), P(C.c_int32),
+                                             P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
+            "or_elimtree_merge_check": (C.c_int, [C.c_int32, P(C.c_uint64), P(C.c_int32)]),
+            "or_dyncpu": (C.c_uint32, [C.c_int32, P(C.c_uint64), P(C.c_int32), P(C.c_int32)]),
+            "or_dyncpu_subset_td": (C.c_uint64, [C.c_int32, P(C.c_uint64), C.c_int32, C.c_uint64]),
+            "or_dyncpu_improv": (C.c_uint32, [C.c_int32, P(C.c_uint64), P(C.c_int32), P(C.c_int32)]),
+            "or_dp_fast": (C.c_int32, [C.c_int32, P(C.c_uint64), P(C.c_uint8), P(C.c_int8), C.c_int32]),
+            "or_certify": (C.c_int, [C.c_int32, P(C.c_uint64), P(C.c_int8), C.c_int32]),
+            "or_brute_td": (C.c_int32, [C.c_int32, P(C.c_uint64)]),
+            "or_bnb": (C.c_int32, [C.c_int32, P(C.c_uint64), C.c_int32, C.c_int32, C.c_uint64,
+                                   P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_uint64)]),
+            "or_heuristic": (C.c_int32, [C.c_int32, P(C.c_uint64), C.c_int32, P(C.c_int32),
+                                         P(C.c_int32)]),
+            "or_edge_lb_double": (C.c_uint32, [C.c_int32, C.c_int32]),
+            "or_exact_memo": (C.c_int32, [C.c_int32, P(C.c_uint64), C.c_uint64, P(C.c_int8),
+                                          P(C.c_uint64)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        self.L = L
+        self.ref = None
+        if REF_SO.exists():
+            R = C.CDLL(str(REF_SO))
+            R.ref_nchoosek.restype = C.c_uint64
+            R.ref_nchoosek.argtypes = [C.c_uint64, C.c_uint64]
+            R.ref_encode.restype = C.c_uint64
+            R.ref_encode.argtypes = [P(C.c_int32), C.c_int32]
+            R.ref_encode_exclude.restype = C.c_uint64
+            R.ref_encode_exclude.argtypes = [P(C.c_int32), C.c_int32, C.c_int32]
+            R.ref_decode.restype = None
+            R.ref_decode.argtypes = [C.c_int32, C.c_int32, C.c_uint64, P(C.c_int32)]
+            self.ref = R
+
+    # ---- colex -----------------------------------------------------------
+    def nchoosek(self, n, k):
+        return self.L.or_nchoosek(n, k)
+
+    def encode(self, s):
+        a = np.array(sorted(s), np.int32)
+        return self.L.or_encode(_p(a, C.c_int32), len(a))
+
+    def encode_exclude(self, s, e):
+        a = np.array(s, np.int32)
+        return self.L.or_encode_exclude(_p(a, C.c_int32), len(a), e)
+
+    def decode(self, n, k, code):
+        out = np.zeros(max(k, 1), np.int32)
+        self.L.or_decode(n, k, code, _p(out, C.c_int32))
+        return out[:k].tolist()
+
+    # ---- elimination tree -------------------------------------------------
+    def elimtree_replay(self, g, order):
+        n = g.n
+        adj = g.adj_array()
+        o = np.array(order, np.int32)
+        masks = np.zeros((n + 1) * n, np.uint64)
+        depths = np.zeros((n + 1) * n, np.int32)
+        counts = np.zeros(n + 1, np.int32)
+        parent = np.zeros(n, np.int32)
+        td = C.c_int32()
+        root = C.c_int32()
+        self.L.or_elimtree_replay(n, _p(adj, C.c_uint64), _p(o, C.c_int32), _p(masks, C.c_uint64),
+                                  _p(depths, C.c_int32), _p(counts, C.c_int32),
+                                  _p(parent, C.c_int32), C.byref(td), C.byref(root))
+        steps = []
+        for i in range(n + 1):
+            steps.append([(int(masks[i * n + j]), int(depths[i * n + j])) for j in range(counts[i])])
+        return steps, parent.tolist(), td.value, root.value
+
+    def elimtree_merge_check(self, g, order):
+        adj = g.adj_array()
+        o = np.array(order, np.int32)
+        return self.L.or_elimtree_merge_check(g.n, _p(adj, C.c_uint64), _p(o, C.c_int32))
+
+    # ---- DPs -----------------------------------------------------------------
+    def dyncpu(self, g):
+        adj = g.adj_array()
+        par = np.zeros(g.n, np.int32)
+        root = C.c_int32()
+        td = self.L.or_dyncpu(g.n, _p(adj, C.c_uint64), _p(par, C.c_int32), C.byref(root))
+        return td, par.tolist(), root.value
+
+    def dyncpu_subset_td(self, g, k, code):
+        adj = g.adj_array()
+        return self.L.or_dyncpu_subset_td(g.n, _p(adj, C.c_uint64), k, code)
+
+    def dyncpu_improv(self, g):
+        adj = g.adj_array()
+        par = np.zeros(g.n, np.int32)
+        root = C.c_int32()
+        td = self.L.or_dyncpu_improv(g.n, _p(adj, C.c_uint64), _p(par, C.c_int32), C.byref(root))
+        return td, par.tolist(), root.value
+
+    def dp_fast(self, g, levels=False, threads=0):
+        adj = g.adj_array()
+        par = np.zeros(g.n, np.int8)
+        lv = np.zeros(1 << g.n, np.uint8) if levels else None
+        td = self.L.or_dp_fast(g.n, _p(adj, C.c_uint64),
+                               _p(lv, C.c_uint8) if levels else None,
+                               _p(par, C.c_int8), threads)
+        if levels:
+            return td, par.astype(int).tolist(), lv
+        return td, par.astype(int).tolist()
+
+    def certify(self, g, parent, td):
+        adj = g.adj_array()
+        p = np.array(parent, np.int8)
+        return bool(self.L.or_certify(g.n, _p(adj, C.c_uint64), _p(p, C.c_int8), td))
+
+    def brute_td(self, g):
+        adj = g.adj_array()
+        return self.L.or_brute_td(g.n, _p(adj, C.c_uint64))
+
+    def bnb(self, g, lb_kind=0, heur_chain=1, node_budget=0):
+        adj = g.adj_array()
+        par = np.zeros(g.n, np.int32)
+        root = C.c_int32()
+        cert = C.c_int32()
+        nodes = C.c_uint64()
+        td = self.L.or_bnb(g.n, _p(adj, C.c_uint64), lb_kind, heur_chain, node_budget,
+                           _p(par, C.c_int32), C.byref(root), C.byref(cert), C.byref(nodes))
+        return td, par.tolist(), root.value, bool(cert.value), nodes.value
+
+    def heuristic(self, g, chain):
+        adj = g.adj_array()
+        par = np.zeros(g.n, np.int32)
+        root = C.c_int32()
+        td = self.L.or_heuristic(g.n, _p(adj, C.c_uint64), chain, _p(par, C.c_int32), C.byref(root))
+        return td, par.tolist(), root.value
+
+    def edge_lb(self, n, e):
+        return self.L.or_edge_lb_double(n, e)
+
+    def exact(self, g, node_budget=0):
+        adj = g.adj_array()
+        par = np.zeros(g.n, np.int8)
+        nodes = C.c_uint64()
+        td = self.L.or_exact_memo(g.n, _p(adj, C.c_uint64), node_budget, _p(par, C.c_int8),
+                                  C.byref(nodes))
+        return td, par.astype(int).tolist(), nodes.value

@@ -1,0 +1,262 @@
+"""GPU parity of the subset DP (td_dp_* in include/tdhip.h) against the oracle.
+
+Bit-exact on treedepth, on every level table, and on the canonical
+decomposition; size-independent checks at n = 30.
+"""
+import json
+import math
+import random
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from gisbdw_amd import api
+from gisbdw_amd import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "dp_golden.json"
+
+
+def level_offsets(n):
+    off = [0]
+    for k in range(n + 1):
+        off.append(off[-1] + math.comb(n, k))
+    return off
+
+
+def mask_of(vs):
+    m = 0
+    for v in vs:
+        m |= 1 << v
+    return m
+
+
+def colex_rank(m):
+    r, i = 0, 0
+    while m:
+        x = (m & -m).bit_length() - 1
+        m &= m - 1
+        i += 1
+        r += math.comb(x, i)
+    return r
+
+
+def components(adj, s):
+    out = []
+    while s:
+        c = s & -s
+        f = c
+        while f:
+            u = (f & -f).bit_length() - 1
+            f &= f - 1
+            nb = adj[u] & s & ~c
+            c |= nb
+            f |= nb
+        out.append(c)
+        s &= ~c
+    return out
+
+
+def ref_elimination(g, levels, S, steps):
+    """GetElimination semantics (dynamic_gpu.hpp:49-64) on oracle tables:
+    connected -> smallest v with td(S\\v) = td(S)-1; disconnected -> root of
+    the first component of largest td (dynamic_gpu.cu:90-104)."""
+    off = level_offsets(g.n)
+
+    def td(m):
+        k = bin(m).count("1")
+        return int(levels[off[k] + colex_rank(m)])
+
+    out = []
+    for _ in range(steps):
+        comps = components(g.adj, S)
+        best = max(td(c) for c in comps)
+        C = next(c for c in comps if td(c) == best)
+        T = td(C)
+        root = next(v for v in range(g.n) if C >> v & 1 and td(C & ~(1 << v)) + 1 == T)
+        out.append(root)
+        S &= ~(1 << root)
+    return out
+
+
+@pytest.fixture(scope="module")
+def dp(ctx):
+    return api.DynamicGPU(ctx)
+
+
+def test_golden_cases(dp, oracle):
+    cases = json.loads(GOLDEN.read_text())["cases"]
+    for c in cases:
+        g = G.Graph.from_edges(c["n"], c["edges"], c["name"])
+        print("golden", c["name"], flush=True)
+        dp(g)
+        assert dp.treedepth == c["treedepth"], c["name"]
+        assert dp.parent == c["parent"], c["name"]
+        assert oracle.certify(g, dp.parent, dp.treedepth)
+
+
+@pytest.mark.parametrize("low_bits", [2, 3, 4, 5])
+def test_all_levels_bit_exact(ctx, oracle, low_bits):
+    ctx.set_param("dp.low_bits", low_bits)
+    try:
+        dp = api.DynamicGPU(ctx)
+        rng = random.Random(1234 + low_bits)
+        for trial in range(12):
+            n = rng.randint(1, 16)
+            p = rng.choice([0.1, 0.2, 0.35, 0.6, 0.9])
+            g = G.gnp(n, p, rng.getrandbits(32))
+            print("levels", low_bits, n, p, flush=True)
+            td, par, lv = oracle.dp_fast(g, levels=True)
+            dp(g)
+            assert dp.treedepth == td and dp.parent == par
+            off = level_offsets(n)
+            for k in range(n + 1):
+                got = dp.level(k)
+                assert np.array_equal(got, lv[off[k]: off[k + 1]]), (n, k)
+    finally:
+        ctx.set_param("dp.low_bits", 4)
+
+
+def test_queries_match_reference_semantics(dp, oracle):
+    g = G.gnp_connected(13, 0.3, 5)
+    td, par, lv = oracle.dp_fast(g, levels=True)
+    dp(g)
+    rng = random.Random(7)
+    off = level_offsets(g.n)
+    for _ in range(60):
+        k = rng.randint(0, g.n)
+        code = rng.randrange(math.comb(g.n, k))
+        assert dp.GetTreedepth(g.n, k, code) == lv[off[k] + code]
+        S = mask_of(oracle.decode(g.n, k, code))
+        assert dp.GetElimination(g.n, k, code) == ref_elimination(g, lv, S, k)
+        # the faithful DynamicCPU stores only connected subsets
+        want = oracle.dyncpu_subset_td(g, k, code) if k <= 6 else None
+        if want is not None and want != 2**64 - 1:
+            assert want == lv[off[k] + code]
+    # the whole graph: GetElimination(n, n, 0) replayed gives the tree
+    order = dp.GetElimination(g.n, g.n, 0)
+    steps, parent, tdd, root = oracle.elimtree_replay(g, order)
+    assert parent == dp.parent and tdd == dp.treedepth
+
+
+def test_config1_graph(dp, oracle):
+    g = G.workload_dp_single(0)
+    td, par = oracle.dp_fast(g)
+    dp(g)
+    assert (dp.treedepth, dp.parent) == (td, par)
+    st = dp.stats
+    assert st.subsets == 1 << 20
+    assert st.alg_bytes == (1 << 19) * 22
+
+
+@pytest.mark.parametrize("graph", ["gnp24", "grid"])
+def test_n24_vs_oracle(dp, oracle, graph):
+    g = G.gnp(24, 0.25, 11) if graph == "gnp24" else G.grid_dropped(4, 6, 0.1, 3)
+    print("n24", graph, flush=True)
+    td, par, lv = oracle.dp_fast(g, levels=True)
+    dp(g)
+    assert (dp.treedepth, dp.parent) == (td, par)
+    off = level_offsets(g.n)
+    for k in (1, 5, 12, 23, 24):
+        assert np.array_equal(dp.level(k), lv[off[k]: off[k + 1]])
+
+
+def check_level_recurrence(g, prev, cur, k, samples, rng):
+    """td(S) from td(S \\ v) by the textbook recurrence, on sampled S."""
+    n = g.n
+    for _ in range(samples):
+        r = rng.randrange(math.comb(n, k))
+        # unrank
+        S, rr, kk = 0, r, k
+        for x in range(n - 1, -1, -1):
+            c = math.comb(x, kk)
+            if kk > 0 and rr >= c:
+                S |= 1 << x
+                rr -= c
+                kk -= 1
+        comps = components(g.adj, S)
+        vals = [int(prev[colex_rank(S & ~(1 << v))]) for v in range(n) if S >> v & 1]
+        if len(comps) == 1:
+            want = 1 + min(vals)
+        else:
+            want = max(vals)
+        assert int(cur[r]) == want, (k, r)
+
+
+@pytest.mark.parametrize("which", [0, 4])
+def test_n30_properties(dp, oracle, which):
+    """Full-size: certified tree, depth = td, and the recurrence holds on
+    sampled subsets of the middle levels read back from the device."""
+    g = G.workload_dp_n30()[which]
+    print("n30", g.name, flush=True)
+    dp(g)
+    assert oracle.certify(g, [int(p) for p in dp.parent], dp.treedepth)
+    rng = random.Random(99)
+    for k in (2, 14, 15, 29, 30):
+        prev = dp.level(k - 1)
+        cur = dp.level(k)
+        check_level_recurrence(g, prev, cur, k, 300, rng)
+    assert int(dp.level(30)[0]) == dp.treedepth
+
+
+def test_batch_modes_equal_single(ctx, oracle):
+    rng = random.Random(5)
+    graphs = [G.gnp(rng.randint(6, 18), rng.uniform(0.15, 0.5), rng.getrandbits(32))
+              for _ in range(40)]
+    want = [oracle.dp_fast(g) for g in graphs]
+    for mode in (0, 1, 2):
+        print("batch mode", mode, flush=True)
+        tds, pars, st = api.dp_solve_batch(ctx, graphs, mode)
+        for (td, par), t, p in zip(want, tds, pars):
+            assert (t, p) == (td, par)
+        assert st.graphs == len(graphs)
+
+
+def test_graph_capture_replay_equal(ctx, oracle):
+    dp = api.DynamicGPU(ctx)
+    gs = [G.gnp(22, 0.3, s) for s in range(3)]
+    ctx.set_param("dp.use_graph", 1)
+    first = []
+    for rep in range(3):
+        for g in gs:
+            dp(g)
+            first.append((dp.treedepth, dp.parent, dp.level(11).tobytes()))
+    ctx.set_param("dp.use_graph", 0)
+    for i, g in enumerate(gs):
+        dp(g)
+        assert (dp.treedepth, dp.parent, dp.level(11).tobytes()) == first[i]
+    for i in range(len(first)):
+        assert first[i] == first[i % len(gs)]
+    ctx.set_param("dp.use_graph", 1)
+
+
+def test_errors(ctx):
+    dp = api.DynamicGPU(ctx)
+    bad = G.Graph(3, [0b010, 0, 0])  # 0-1 listed only in row 0
+    with pytest.raises(api.TDError) as e:
+        dp(bad)
+    assert e.value.code == api.TD_E_INVALID
+    loop = G.Graph(2, [0b01, 0])
+    with pytest.raises(api.TDError):
+        dp(loop)
+    with pytest.raises(api.TDError):
+        dp(G.path(33))
+    small = api.Context(mem_budget_bytes=1 << 16)
+    with pytest.raises(api.TDError) as e:
+        api.DynamicGPU(small)(G.gnp(24, 0.3, 1))
+    assert e.value.code == api.TD_E_NOMEM
+    fresh = api.DynamicGPU(api.Context())
+    with pytest.raises(api.TDError) as e:
+        fresh.GetTreedepth(0, 0, 0)
+    assert e.value.code in (api.TD_E_STATE, api.TD_E_INVALID)
+
+
+def test_deterministic(dp):
+    g = G.gnp(22, 0.35, 8)
+    dp(g)
+    a = [dp.level(k).tobytes() for k in range(23)]
+    dp(g)
+    b = [dp.level(k).tobytes() for k in range(23)]
+    assert a == b
