@@ -1,0 +1,95 @@
+"""Host-side logic that needs no GPU: generators, text format, and the C ABI
+library's exported symbols."""
+import ctypes as C
+import math
+import re
+from pathlib import Path
+
+import pytest
+
+from gisbdw_amd import graphs as G
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def test_generators_deterministic_and_valid():
+    for f in (lambda s: G.gnp(20, 0.3, s), lambda s: G.random_cubic(40, s),
+              lambda s: G.tree_plus_edges(50, 0.1, s), lambda s: G.grid_dropped(5, 6, 0.1, s)):
+        a, b = f(3), f(3)
+        assert a.adj == b.adj
+        a.validate()
+    assert G.gnp(20, 0.3, 1).adj != G.gnp(20, 0.3, 2).adj
+
+
+def test_family_shapes():
+    g = G.random_cubic(48, 1)
+    assert all(g.degree(v) == 3 for v in range(48)) and g.is_connected()
+    t = G.tree_plus_edges(41, 0.1, 5)
+    assert t.m == 40 + math.ceil(0.1 * 40) and t.is_connected()
+    gr = G.grid_dropped(5, 6, 0.1, 0)
+    assert G.grid(5, 6).m == 49 and gr.m == 49 - 5
+    assert G.gnp_connected(20, 0.3, 0).is_connected()
+    assert G.chordal_cycle(12).m == 14
+    h = G.halin(14, 3)
+    assert h.is_connected()
+
+
+def test_workloads():
+    ws = G.workload_dp_n30()
+    assert len(ws) == 8 and all(g.n == 30 for g in ws)
+    b = G.workload_dp_batch(64, 0)
+    assert all(12 <= g.n <= 24 for g in b)
+    bb = G.workload_bnb(16, 0)
+    assert all(40 <= g.n <= 64 for g in bb) and all(g.is_connected() for g in bb)
+
+
+def test_text_format_roundtrip():
+    g = G.gnp(15, 0.3, 4)
+    h = G.parse_graph_text(G.format_graph_text(g))
+    assert h.n == g.n and h.adj == g.adj
+    # the reference's edge syntax, with trailing isolated vertices kept
+    t = "graph {\n0;\n1;\n2;\n3;\n0 -- 1;\n1 -- 2;\n}\n"
+    h = G.parse_graph_text(t)
+    assert h.n == 4 and h.edges() == [(0, 1), (1, 2)]
+
+
+def test_validation_rejects_bad_graphs():
+    with pytest.raises(ValueError):
+        G.Graph(2, [0b01, 0]).validate()
+    with pytest.raises(ValueError):
+        G.Graph(3, [0b010, 0, 0]).validate()
+    with pytest.raises(ValueError):
+        G.Graph(2, [0b100, 0]).validate()
+
+
+def header_functions():
+    text = (ROOT / "include" / "tdhip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(td_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from gisbdw_amd import api
+
+    lib_path = api.library_path()
+    if not lib_path.exists():
+        pytest.skip("libtdhip.so not built")
+    lib = C.CDLL(str(lib_path))
+    names = header_functions()
+    assert "td_dp_solve" in names and "td_ctx_create" in names
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_product_path_fails_loudly_without_gpu():
+    """No CPU fallback: creating a context without a usable device raises."""
+    from gisbdw_amd import api
+
+    if not api.library_path().exists():
+        pytest.skip("libtdhip.so not built")
+    import os
+
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU may be present")
+    with pytest.raises(api.TDError):
+        api.Context()

@@ -1,0 +1,328 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Each check is chosen so a plausible mistake (dropped term, wrong index,
+transposed operand) fails it: closed forms, the reference's own
+known-answer tests (transcribed as data), the reference's own colex codec
+compiled from its source (oracle/_ref), a definition brute force, and
+agreement between independently written recurrences.
+"""
+import itertools
+import math
+import random
+
+import numpy as np
+import pytest
+
+from gisbdw_amd import graphs as G
+
+
+# ---------------------------------------------------------------------------
+# colex codec -- set_encoder_unittest.cpp:16-116 (n in [0, 16))
+# ---------------------------------------------------------------------------
+def test_nchoosek_pascal_and_closed_form(oracle):
+    for n in range(0, 40):
+        assert oracle.nchoosek(n, 0) == 1
+        assert oracle.nchoosek(n, n) == 1
+        for k in range(n + 1, n + 10):
+            assert oracle.nchoosek(n, k) == 0
+        for k in range(1, n):
+            assert oracle.nchoosek(n, k) == oracle.nchoosek(n - 1, k - 1) + oracle.nchoosek(n - 1, k)
+            assert oracle.nchoosek(n, k) == math.comb(n, k)
+
+
+@pytest.mark.parametrize("n", range(0, 16))
+def test_encode_decode_bijection(oracle, n):
+    for k in range(n + 1):
+        seen = set()
+        for code in range(math.comb(n, k)):
+            s = oracle.decode(n, k, code)
+            assert len(s) == k and all(0 <= v < n for v in s)
+            assert s == sorted(s)
+            assert oracle.encode(s) == code
+            seen.add(tuple(s))
+        assert len(seen) == math.comb(n, k)
+
+
+@pytest.mark.parametrize("n", [5, 9, 13])
+def test_encode_exclude(oracle, n):
+    for k in range(n + 1):
+        for code in range(math.comb(n, k)):
+            s = oracle.decode(n, k, code)
+            for i in range(k):
+                assert oracle.encode_exclude(s, i) == oracle.encode(s[:i] + s[i + 1:])
+
+
+def test_colex_order_is_numeric_order(oracle):
+    """Consecutive codes are consecutive k-bit masks in numeric order, the
+    property the GPU tiles use to step ranks with Gosper's successor."""
+    n = 12
+    for k in range(1, n):
+        masks = []
+        for code in range(math.comb(n, k)):
+            m = 0
+            for v in oracle.decode(n, k, code):
+                m |= 1 << v
+            masks.append(m)
+        want = sorted(m for m in range(1 << n) if bin(m).count("1") == k)
+        assert masks == want
+
+
+def test_codec_matches_reference_build(oracle):
+    """The reference's own set_encoder, compiled from its source into
+    oracle/_ref by oracle/Makefile."""
+    if oracle.ref is None:
+        pytest.skip("oracle/_ref/libref_set_encoder.so not built (reference tree absent)")
+    import ctypes as C
+
+    R = oracle.ref
+    for n in range(0, 20):
+        for k in range(0, n + 2):
+            assert R.ref_nchoosek(n, k) == oracle.nchoosek(n, k)
+    rng = random.Random(3)
+    for n in range(1, 24):
+        for _ in range(200):
+            k = rng.randint(0, n)
+            code = rng.randrange(math.comb(n, k))
+            out = np.zeros(max(k, 1), np.int32)
+            R.ref_decode(n, k, code, out.ctypes.data_as(C.POINTER(C.c_int32)))
+            assert out[:k].tolist() == oracle.decode(n, k, code)
+            s = np.array(oracle.decode(n, k, code), np.int32)
+            p = s.ctypes.data_as(C.POINTER(C.c_int32))
+            assert R.ref_encode(p, k) == code
+            for e in range(k):
+                assert R.ref_encode_exclude(p, k, e) == oracle.encode_exclude(s.tolist(), e)
+
+
+# ---------------------------------------------------------------------------
+# EliminationTree -- elimination_tree_unittest.cpp:29-270 (data transcribed)
+# ---------------------------------------------------------------------------
+def M(*vs):
+    m = 0
+    for v in vs:
+        m |= 1 << v
+    return m
+
+
+KATS = [
+    # (graph, order, components after each step as {(mask, depth)}, tree edges, depth)
+    (
+        G.cycle(5), [0, 3, 2, 4, 1],
+        [
+            {(M(0, 1, 2, 3, 4), 0)},
+            {(M(1, 2, 3, 4), 1)},
+            {(M(4), 2), (M(1, 2), 2)},
+            {(M(4), 2), (M(1), 3)},
+            {(M(1), 3)},
+            set(),
+        ],
+        {(0, 3), (2, 3), (1, 2), (3, 4)}, 4,
+    ),
+    (
+        G.Graph.from_edges(7, [(0, 1), (1, 2), (2, 0), (2, 3), (3, 6), (6, 5), (5, 4), (4, 6)]),
+        [3, 6, 4, 5, 2, 1, 0],
+        [
+            {(M(0, 1, 2, 3, 4, 5, 6), 0)},
+            {(M(0, 1, 2), 1), (M(4, 5, 6), 1)},
+            {(M(0, 1, 2), 1), (M(4, 5), 2)},
+            {(M(0, 1, 2), 1), (M(5), 3)},
+            {(M(0, 1, 2), 1)},
+            {(M(0, 1), 2)},
+            {(M(0), 3)},
+            set(),
+        ],
+        {(2, 3), (3, 6), (4, 6), (4, 5), (1, 2), (0, 1)}, 4,
+    ),
+    (
+        G.path(4), [3, 2, 1, 0],
+        [
+            {(M(0, 1, 2, 3), 0)},
+            {(M(0, 1, 2), 1)},
+            {(M(0, 1), 2)},
+            {(M(0), 3)},
+            set(),
+        ],
+        {(0, 1), (1, 2), (2, 3)}, 4,
+    ),
+    (
+        G.path(7), [3, 1, 5, 0, 2, 4, 6],
+        [
+            {(M(0, 1, 2, 3, 4, 5, 6), 0)},
+            {(M(0, 1, 2), 1), (M(4, 5, 6), 1)},
+            {(M(0), 2), (M(2), 2), (M(4, 5, 6), 1)},
+            {(M(0), 2), (M(2), 2), (M(4), 2), (M(6), 2)},
+            {(M(2), 2), (M(4), 2), (M(6), 2)},
+            {(M(4), 2), (M(6), 2)},
+            {(M(6), 2)},
+            set(),
+        ],
+        {(1, 3), (3, 5), (0, 1), (1, 2), (4, 5), (5, 6)}, 3,
+    ),
+]
+
+
+@pytest.mark.parametrize("case", range(len(KATS)))
+def test_elimination_tree_kats(oracle, case):
+    g, order, comps, edges, depth = KATS[case]
+    steps, parent, td, root = oracle.elimtree_replay(g, order)
+    for i, want in enumerate(comps):
+        assert set(steps[i]) == want, i
+        lows = [m & -m for m, _ in steps[i]]
+        assert lows == sorted(lows)  # ComponentCmp order
+    got_edges = {(min(v, p), max(v, p)) for v, p in enumerate(parent) if p >= 0}
+    assert got_edges == {(min(a, b), max(a, b)) for a, b in edges}
+    assert td == depth
+    assert root == order[0]
+    assert oracle.elimtree_merge_check(g, order) == 0
+
+
+# ---------------------------------------------------------------------------
+# DP values: closed forms, brute force, three independent recurrences
+# ---------------------------------------------------------------------------
+CLOSED = (
+    [(G.path(n), math.ceil(math.log2(n + 1))) for n in range(1, 22)]
+    + [(G.cycle(n), 1 + math.ceil(math.log2(n))) for n in range(3, 20)]
+    + [(G.complete(n), n) for n in range(1, 12)]
+    + [(G.star(n), 2) for n in range(3, 14)]
+    + [(G.complete_bipartite(a, b), min(a, b) + 1) for a in range(1, 5) for b in range(1, 6)]
+    + [(G.edgeless(n), 1) for n in (1, 4, 9)]
+)
+
+
+@pytest.mark.parametrize("idx", range(len(CLOSED)))
+def test_closed_forms(oracle, idx):
+    g, want = CLOSED[idx]
+    td, par = oracle.dp_fast(g)
+    assert td == want, g.name
+    assert oracle.certify(g, par, td)
+    if g.n <= 14 and g.is_connected():
+        assert oracle.dyncpu(g)[0] == want
+        assert oracle.dyncpu_improv(g)[0] == want
+
+
+def test_forest_is_max_of_trees(oracle):
+    a, b = G.path(7), G.cycle(9)
+    g = G.Graph(16, a.adj + [r << 7 for r in b.adj])
+    td, par = oracle.dp_fast(g)
+    assert td == max(3, 1 + math.ceil(math.log2(9)))
+    assert oracle.certify(g, par, td)
+    assert sum(1 for p in par if p == -1) == 2
+    # the reference's DynamicCPU does not store a disconnected full set
+    assert oracle.dyncpu(g)[0] == 2**32 - 1
+
+
+def test_brute_force_small(oracle):
+    rng = random.Random(11)
+    for trial in range(60):
+        n = rng.randint(1, 7)
+        g = G.gnp(n, rng.random(), rng.getrandbits(32))
+        bt = oracle.brute_td(g)
+        td, par = oracle.dp_fast(g)
+        assert bt == td, (g.edges(), bt, td)
+
+
+def test_three_recurrences_agree(oracle):
+    rng = random.Random(12)
+    for trial in range(40):
+        n = rng.randint(2, 13)
+        g = G.gnp_connected(n, rng.uniform(0.1, 0.7), rng.getrandbits(32))
+        td, par = oracle.dp_fast(g)
+        td2, par2, r2 = oracle.dyncpu(g)
+        td3, par3, r3 = oracle.dyncpu_improv(g)
+        assert td == td2 == td3
+        assert par == par2 == par3  # the canonical tie-break is shared
+        assert oracle.certify(g, par, td)
+
+
+def test_canonical_root_rule(oracle):
+    """root(C) = the smallest v with td(C \\ v) = td(C) - 1 (strict < over
+    ascending v, dynamic_cpu.cpp:96), checked from the level tables."""
+    g = G.gnp_connected(11, 0.35, 4)
+    td, par, lv = oracle.dp_fast(g, levels=True)
+    off = [0]
+    for k in range(g.n + 1):
+        off.append(off[-1] + math.comb(g.n, k))
+
+    def tdm(m):
+        k = bin(m).count("1")
+        return int(lv[off[k] + oracle.encode([v for v in range(g.n) if m >> v & 1])])
+
+    full = (1 << g.n) - 1
+    root = par.index(-1)
+    assert root == min(v for v in range(g.n) if tdm(full & ~(1 << v)) == td - 1)
+
+
+def test_p3_kat(oracle):
+    """branch_and_bound_unittest.cpp:36-45 / highest_degree_heuristic:
+    P3 -> td 2, root 1, decomposition = P3 itself."""
+    g = G.path(3)
+    td, par, root, cert, _ = oracle.bnb(g)
+    assert (td, root, cert) == (2, 1, True)
+    assert {(min(v, p), max(v, p)) for v, p in enumerate(par) if p >= 0} == {(0, 1), (1, 2)}
+    td, par, root = oracle.heuristic(g, 2)
+    assert (td, root) == (2, 1)
+
+
+def test_certifier_rejects_planted_mistakes(oracle):
+    g = G.cycle(8)
+    td, par = oracle.dp_fast(g)
+    assert oracle.certify(g, par, td)
+    assert not oracle.certify(g, par, td - 1)  # wrong depth
+    bad = list(par)
+    leaf = next(v for v in range(8) if v not in par)
+    bad[leaf] = -1  # a new root breaks an edge's ancestry
+    assert not oracle.certify(g, bad, td)
+    cyc = list(par)
+    r = cyc.index(-1)
+    cyc[r] = leaf  # cycle
+    assert not oracle.certify(g, cyc, td)
+
+
+# ---------------------------------------------------------------------------
+# lower bounds, B&B restatement, exact memo solver
+# ---------------------------------------------------------------------------
+def test_edge_lb_integer_form(oracle):
+    """ceil(0.5 + n - sqrt(0.25 + n^2 - n - 2e)) equals the smallest d with
+    (d-1)(2n-d) >= 2e for every n <= 64, e <= C(n,2)."""
+    for n in range(1, 65):
+        for e in range(0, n * (n - 1) // 2 + 1):
+            d = 1
+            while (d - 1) * (2 * n - d) < 2 * e:
+                d += 1
+            assert oracle.edge_lb(n, e) == d, (n, e)
+
+
+def test_bnb_equals_dp(oracle):
+    rng = random.Random(21)
+    fams = [G.path(n) for n in (7, 8, 10, 12)] + [G.cycle(n) for n in (7, 9, 11)]
+    fams += [G.chordal_cycle(n) for n in (10, 12)]
+    fams += [G.sparse_connected(n, rng.getrandbits(16)) for n in (8, 9, 10, 12, 14)]
+    fams += [G.gnp_connected(rng.randint(6, 12), rng.uniform(0.2, 0.6), rng.getrandbits(16))
+             for _ in range(10)]
+    for g in fams:
+        td, par = oracle.dp_fast(g)
+        for lb_kind in (0, 1):
+            btd, bpar, root, cert, nodes = oracle.bnb(g, lb_kind=lb_kind, heur_chain=1)
+            assert cert and btd == td, (g.name, lb_kind)
+            assert oracle.certify(g, [int(p) for p in bpar], btd)
+        for chain in (1, 2, 3, 4):
+            htd, hpar, _ = oracle.heuristic(g, chain)
+            assert htd >= td
+            assert oracle.certify(g, [int(p) for p in hpar], htd)
+
+
+def test_exact_memo_equals_dp(oracle):
+    rng = random.Random(31)
+    gs = [G.gnp(rng.randint(5, 18), rng.uniform(0.1, 0.5), rng.getrandbits(32)) for _ in range(30)]
+    gs += [G.grid_dropped(4, 5, 0.1, s) for s in range(3)]
+    gs += [G.tree_plus_edges(20, 0.1, s) for s in range(3)]
+    for g in gs:
+        td, par = oracle.dp_fast(g)
+        etd, epar, nodes = oracle.exact(g)
+        assert etd == td, g.name
+        assert oracle.certify(g, epar, etd)
+
+
+def test_bnb_budget_reports_uncertified(oracle):
+    g = G.gnp_connected(14, 0.4, 2)
+    td, par, root, cert, nodes = oracle.bnb(g, lb_kind=1, heur_chain=0, node_budget=5)
+    assert not cert and nodes > 5 - 1
