@@ -98,11 +98,14 @@ def load_library():
                                         P(C.c_uint8), P(C.c_int8), C.c_int32, P(DPStats)]),
         "td_dp_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, P(C.c_double)]),
     }
-    optional = {
+    sig.update({
         "td_bnb_solve_batch": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
                                          P(C.c_int32), P(C.c_float), P(C.c_uint8), P(C.c_int8),
                                          P(C.c_uint8), P(BnBStats)]),
-    }
+        "td_bnb_solve": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_uint64), C.c_int32,
+                                   P(C.c_int32), P(C.c_int8), P(C.c_uint8)]),
+    })
+    optional = {}
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
         f.restype = res
@@ -257,3 +260,56 @@ def dp_solve_batch(ctx: Context, graphs: Sequence[Graph], mode: int = 0):
         parents.append(par[off: off + g.n].astype(int).tolist())
         off += g.n
     return td.astype(int).tolist(), parents, st
+
+
+class BnBGPU:
+    """td::BnBGPU (bnb_gpu.hpp:8-31) on the MI355X branch-and-bound.
+
+    ``BnBGPU(ctx)(g, heur_td)`` returns the treedepth like the reference's
+    ``int operator()(g, heur_td)``; ``solve(g)`` also returns the
+    decomposition and whether the search completed (certified)."""
+
+    def __init__(self, ctx: Optional[Context] = None):
+        self.ctx = ctx or Context()
+        self.lib = self.ctx.lib
+        self.certified = None
+        self.parent = None
+
+    def __call__(self, g: Graph, heur_td: int = 0) -> int:
+        adj = g.adj_array()
+        td = C.c_int32()
+        par = np.zeros(max(g.n, 1), np.int8)
+        cert = C.c_uint8()
+        self.ctx.check(self.lib.td_bnb_solve(self.ctx.h, g.n, _ptr(adj, C.c_uint64), int(heur_td),
+                                             C.byref(td), _ptr(par, C.c_int8), C.byref(cert)))
+        self.parent = par[: g.n].astype(int).tolist()
+        self.certified = bool(cert.value)
+        return td.value
+
+    def solve(self, g: Graph, heur_td: int = 0) -> Tuple[Result, bool]:
+        td = self(g, heur_td)
+        return Result.from_parent(self.parent, td), self.certified
+
+
+def bnb_solve_batch(ctx: Context, graphs: Sequence[Graph], ub_seed=None, scores=None):
+    """Batched exact search.  scores: optional list of per-vertex float arrays
+    (a learned proposal).  Returns (treedepths, parents, certified, stats)."""
+    lib = ctx.lib
+    ns = np.array([g.n for g in graphs], np.int32)
+    adj = np.concatenate([g.adj_array() for g in graphs]).astype(np.uint64)
+    seed = None if ub_seed is None else np.asarray(ub_seed, np.int32)
+    sc = None if scores is None else np.concatenate([np.asarray(s, np.float32) for s in scores])
+    td = np.zeros(len(graphs), np.uint8)
+    cert = np.zeros(len(graphs), np.uint8)
+    par = np.zeros(int(ns.sum()), np.int8)
+    st = BnBStats()
+    ctx.check(lib.td_bnb_solve_batch(
+        ctx.h, len(graphs), _ptr(ns, C.c_int32), _ptr(adj, C.c_uint64),
+        None if seed is None else _ptr(seed, C.c_int32),
+        None if sc is None else _ptr(sc, C.c_float),
+        _ptr(td, C.c_uint8), _ptr(par, C.c_int8), _ptr(cert, C.c_uint8), C.byref(st)))
+    parents, off = [], 0
+    for g in graphs:
+        parents.append(par[off: off + g.n].astype(int).tolist())
+        off += g.n
+    return td.astype(int).tolist(), parents, cert.astype(bool).tolist(), st

@@ -1,0 +1,863 @@
+// bnb.hip -- exact treedepth by branch-and-bound over elimination orderings,
+// one wavefront per graph, lanes = vertices (n <= 64), on gfx950.
+//
+// Reference path: BnBGPU (Treedepth/src/bnb_gpu/bnb_gpu.cu:167-380) and the
+// CPU BranchAndBound it restates (Treedepth/src/branch_and_bound/
+// branch_and_bound.cpp:51-115): choose a root of a component, recurse on the
+// components left behind, prune when depth + lower bound cannot beat the best
+// tree.  Here the search is the decision form
+//   feasible(C, k)  <=>  td(C) <= k      (C connected)
+//   = |C| <= k  or  exists v in C: feasible(D, k-1) for every component D of C-v
+// run for k = UB-1, UB-2, ... until it fails, so the last failure proves
+// td = UB.  Every answer is certified by an exhausted search, never by a
+// heuristic.  Pruning uses proven bounds only:
+//   * |C| <= k (path decomposition),
+//   * EdgeLowerBound in its exact integer form: smallest d with
+//     (d-1)(2|C|-d) >= 2|E(C)| (edge_lower_bound.hpp:13-20),
+//   * k-core: a nonempty k-core of C means td(C) - 1 >= degeneracy >= k,
+//   * path: a BFS from a peripheral vertex exposes a shortest path of e+1
+//     vertices, and td(P_m) = ceil(log2(m+1)),
+//   * a memo in HBM of proven facts per vertex set: lb (td(C) >= lb) and
+//     sol (td(C) <= solk with root r).
+// The wave holds one DFS stack in LDS; every set operation is a 64-bit mask
+// and every neighbourhood is one ballot: lane u votes (adj[u] & F) != 0, so
+// ballot = N(F).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace tdhip {
+namespace {
+
+constexpr int kMaxBnBN = 64;
+constexpr int kStack = 160;
+
+struct BnBGraph {
+  uint32_t n;
+  int32_t ub_seed;   // caller's upper bound (<= 0: none)
+  uint32_t use_scores;
+  uint32_t pad;
+  uint64_t adj[64];
+  float score[64];
+};
+
+struct BnBOut {
+  int32_t td;
+  int32_t lb;        // proven lower bound (== td when certified)
+  uint32_t certified;
+  uint32_t heur_td;  // depth of the initial heuristic tree
+  uint64_t nodes;
+  uint64_t memo_hits;
+  int8_t parent[64];
+};
+
+struct Frame {
+  uint64_t C, cand, pending, curD;
+  int32_t k, v, state, pad;
+};
+
+enum : int { kEnter = 0, kNextRoot = 1, kNextComp = 2, kAfterChild = 3 };
+
+// One wave per workgroup: a workgroup barrier orders lane 0's LDS and
+// global writes before every lane's later reads.
+__device__ __forceinline__ void Sync() { __syncthreads(); }
+
+__device__ __forceinline__ uint64_t Ballot(bool p) {
+  return static_cast<uint64_t>(__ballot(p));
+}
+
+__device__ __forceinline__ uint64_t Uniform64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ int Uniform32(int x) {
+  return __builtin_amdgcn_readfirstlane(x);
+}
+
+// Sum of a per-lane value below 64 over the wave: one ballot per bit, so
+// the six popcounts are independent instead of a chain of shuffles.
+__device__ __forceinline__ uint32_t WaveSum(uint32_t v) {
+  uint32_t t = 0;
+#pragma unroll
+  for (int b = 0; b < 6; ++b)
+    t += static_cast<uint32_t>(__builtin_popcountll(Ballot((v >> b) & 1u))) << b;
+  return t;
+}
+
+__device__ __forceinline__ uint64_t WaveMax64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t o = __shfl_xor(v, off);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Component of `from` inside X (from in X): frontier grows by one ballot a
+// round.  rounds = eccentricity of `from` in the component.
+__device__ __forceinline__ uint64_t Component(uint64_t myadj, uint64_t X,
+                                              uint64_t from, int* rounds,
+                                              uint64_t* last) {
+  uint64_t comp = from, front = from;
+  int r = 0;
+  for (;;) {
+    const uint64_t nb = Ballot((myadj & front) != 0ull) & X & ~comp;
+    if (!nb) break;
+    comp |= nb;
+    front = nb;
+    ++r;
+  }
+  if (rounds) *rounds = r;
+  if (last) *last = front;
+  return comp;
+}
+
+// smallest d with (d-1)(2n-d) >= 2e (EdgeLowerBound in integers)
+__device__ __forceinline__ int EdgeLB(int n, int e) {
+  int d = 1;
+  if (e > 0) {
+    // start near the root of (d-1)(2n-d) = 2e, then settle exactly: the
+    // predicate is monotone in d on [1, n]
+    const float est = n + 0.5f - sqrtf(fmaxf((n - 0.5f) * (n - 0.5f) - 2.0f * e, 0.f));
+    d = min(max(static_cast<int>(est) - 1, 1), max(n, 1));
+    while (d > 1 && (d - 2) * (2 * n - d + 1) >= 2 * e) --d;
+    while ((d - 1) * (2 * n - d) < 2 * e) ++d;
+  }
+  return d;
+}
+
+__device__ __forceinline__ int CeilLog2(int x) {  // x >= 1
+  return x <= 1 ? 0 : 32 - __builtin_clz(static_cast<uint32_t>(x - 1));
+}
+
+// Lower bound on td(C), C connected, all from proven facts.
+__device__ __forceinline__ int LowerBound(uint64_t myadj, bool in_c, uint64_t C,
+                                          int k) {
+  const int n = __builtin_popcountll(C);
+  const int deg = in_c ? __builtin_popcountll(myadj & C) : 0;
+  const int e = static_cast<int>(WaveSum(static_cast<uint32_t>(deg))) / 2;
+  int lb = EdgeLB(n, e);
+  // path: double sweep BFS from the lowest vertex
+  int ecc = 0;
+  uint64_t last = 0;
+  Component(myadj, C, C & (0ull - C), &ecc, &last);
+  Component(myadj, C, last & (0ull - last), &ecc, nullptr);
+  lb = max(lb, CeilLog2(ecc + 2));
+  if (lb > k) return lb;
+  // k-core of C nonempty => degeneracy >= k => td >= k+1
+  if (k >= 1) {
+    uint64_t X = C;
+    for (;;) {
+      const bool in = (X >> (threadIdx.x & 63)) & 1ull;
+      const int d = in ? __builtin_popcountll(myadj & X) : 0;
+      const uint64_t keep = Ballot(in && d >= k);
+      if (keep == X) break;
+      X = keep;
+    }
+    if (X) lb = max(lb, k + 1);
+  }
+  return lb;
+}
+
+// degeneracy of X: largest d whose d-core is nonempty (parallel peeling)
+__device__ __forceinline__ int Degeneracy(uint64_t myadj, uint64_t X0) {
+  const int lane = threadIdx.x & 63;
+  int d = 0;
+  for (int t = 1; t < 64; ++t) {
+    uint64_t X = X0;
+    for (;;) {
+      const bool in = (X >> lane) & 1ull;
+      const int dd = in ? __builtin_popcountll(myadj & X) : 0;
+      const uint64_t keep = Ballot(in && dd >= t);
+      if (keep == X) break;
+      X = keep;
+    }
+    if (!X) break;
+    d = t;
+  }
+  return d;
+}
+
+// ---- memo: per-wave table of 2^log2 buckets x 4 entries of 16 bytes -------
+// entry = {key lo, key hi, data, tag}; data: bits 0-7 lb, 8-15 solk (0 =
+// none), 16-23 root; tag = graph serial + 1 (0 = empty), so a new graph needs
+// no clearing.
+struct Memo {
+  uint4* table;
+  uint32_t mask;  // buckets - 1
+  uint32_t tag;
+};
+
+__device__ __forceinline__ uint32_t BucketOf(uint64_t C, uint32_t mask) {
+  return static_cast<uint32_t>((C * 0x9E3779B97F4A7C15ull) >> 40) & mask;
+}
+
+// Probe in two halves so the load's latency overlaps other work: MemoIssue
+// loads C's bucket (lanes 0-3, one entry each; the workgroup barrier first
+// retires this wave's earlier entry stores); MemoResolve returns the entry's
+// data (0 if absent) and its slot, or -1-victim when absent.
+__device__ __forceinline__ uint4 MemoIssue(const Memo& m, uint64_t C) {
+  Sync();
+  const int lane = threadIdx.x & 63;
+  uint4 e = make_uint4(0, 0, 0, 0);
+  if (lane < 4) e = m.table[static_cast<size_t>(BucketOf(C, m.mask)) * 4 + lane];
+  return e;
+}
+
+__device__ __forceinline__ uint32_t MemoResolve(const Memo& m, uint64_t C,
+                                                uint4 e, int* slot) {
+  const int lane = threadIdx.x & 63;
+  const bool hit = lane < 4 && e.w == m.tag && e.x == static_cast<uint32_t>(C) &&
+                   e.y == static_cast<uint32_t>(C >> 32);
+  const uint64_t hb = Ballot(hit);
+  if (hb) {
+    const int s = __builtin_ctzll(hb);
+    *slot = s;
+    return static_cast<uint32_t>(__shfl(static_cast<int>(e.z), s));
+  }
+  const uint64_t fb = Ballot(lane < 4 && e.w != m.tag);
+  *slot = fb ? -1 - __builtin_ctzll(fb)
+             : -1 - static_cast<int>((C >> 40) & 3ull);  // victim
+  return 0u;
+}
+
+__device__ __forceinline__ uint32_t MemoProbe(const Memo& m, uint64_t C,
+                                              int* slot) {
+  const uint4 e = MemoIssue(m, C);
+  return MemoResolve(m, C, e, slot);
+}
+
+__device__ __forceinline__ void MemoStore(const Memo& m, uint64_t C, int slot,
+                                          uint32_t old, uint32_t lb, uint32_t solk,
+                                          uint32_t root) {
+  uint32_t olb = old & 0xFF, osk = (old >> 8) & 0xFF, ort = (old >> 16) & 0xFF;
+  if (slot < 0) {  // new entry
+    olb = 0;
+    osk = 0;
+    ort = 0;
+  }
+  const uint32_t nlb = max(olb, lb);
+  uint32_t nsk = osk, nrt = ort;
+  if (solk && (!osk || solk < osk)) {
+    nsk = solk;
+    nrt = root;
+  }
+  const int s = slot < 0 ? -1 - slot : slot;
+  if ((threadIdx.x & 63) == 0)
+    m.table[static_cast<size_t>(BucketOf(C, m.mask)) * 4 + s] =
+        make_uint4(static_cast<uint32_t>(C), static_cast<uint32_t>(C >> 32),
+                   nlb | (nsk << 8) | (nrt << 16), m.tag);
+}
+
+// Key for choosing the next root: plain = degree inside C (ties: lower id);
+// learned = the vertex score (ties: lower id).
+__device__ __forceinline__ uint64_t RootKey(bool use_scores, float score,
+                                           uint64_t myadj, uint64_t C, int lane) {
+  uint32_t primary;
+  if (use_scores) {
+    uint32_t u = __float_as_uint(score);
+    primary = (u & 0x80000000u) ? ~u : (u | 0x80000000u);  // order-preserving
+  } else {
+    primary = static_cast<uint32_t>(__builtin_popcountll(myadj & C));
+  }
+  return (static_cast<uint64_t>(primary) << 8) | static_cast<uint64_t>(63 - lane);
+}
+
+__device__ __forceinline__ int PickRoot(bool use_scores, float score,
+                                        uint64_t myadj, uint64_t C,
+                                        uint64_t cand) {
+  const int lane = threadIdx.x & 63;
+  const bool in = (cand >> lane) & 1ull;
+  const uint64_t key = in ? RootKey(use_scores, score, myadj, C, lane) + 1 : 0;
+  const uint64_t best = WaveMax64(key);
+  return __builtin_ctzll(Ballot(in && key == best));
+}
+
+struct Search {
+  Frame* st;  // LDS stack
+  uint64_t myadj;
+  float score;
+  bool use_scores;
+  Memo memo;
+  uint64_t nodes, budget, hits;
+  bool aborted;
+};
+
+// Decision search td(C0) <= k0 ?  Uniform across the wave.
+__device__ bool Feasible(Search& S, uint64_t C0, int k0) {
+  const int lane = threadIdx.x & 63;
+  int sp = 0;
+  bool ret = false;
+  auto push = [&](uint64_t C, int k) {
+    if (lane == 0) {
+      Frame f;
+      f.C = C;
+      f.k = k;
+      f.state = kEnter;
+      f.cand = f.pending = f.curD = 0;
+      f.v = -1;
+      f.pad = 0;
+      S.st[sp] = f;
+    }
+    ++sp;
+    Sync();
+  };
+  push(C0, k0);
+  while (sp > 0) {
+    Frame* F = &S.st[sp - 1];
+    const uint64_t C = Uniform64(F->C);
+    const int k = Uniform32(F->k);
+    const int state = Uniform32(F->state);
+    if (state == kEnter) {
+      const int n = __builtin_popcountll(C);
+      if (n <= k) { ret = true; --sp; continue; }
+      if (k <= 0) { ret = false; --sp; continue; }
+      if (S.aborted) { ret = false; --sp; continue; }
+      ++S.nodes;
+      if (S.budget && S.nodes > S.budget) {
+        S.aborted = true;
+        ret = false;
+        --sp;
+        continue;
+      }
+      // the memo load is in flight while the bounds are computed
+      const uint4 me = MemoIssue(S.memo, C);
+      const bool in_c = (C >> lane) & 1ull;
+      const int lb = LowerBound(S.myadj, in_c, C, k);
+      int slot;
+      const uint32_t md = MemoResolve(S.memo, C, me, &slot);
+      if (slot >= 0) {
+        const uint32_t mlb = md & 0xFF, sk = (md >> 8) & 0xFF;
+        if (sk && sk <= static_cast<uint32_t>(k)) { ++S.hits; ret = true; --sp; continue; }
+        if (mlb > static_cast<uint32_t>(k)) { ++S.hits; ret = false; --sp; continue; }
+      }
+      if (lb > k) {
+        MemoStore(S.memo, C, slot, md, static_cast<uint32_t>(lb), 0, 0);
+        ret = false;
+        --sp;
+        continue;
+      }
+      if (lane == 0) {
+        F->cand = C;
+        F->state = kNextRoot;
+      }
+      Sync();
+      continue;
+    }
+    if (state == kNextRoot) {
+      const uint64_t cand = Uniform64(F->cand);
+      if (cand == 0 || S.aborted) {
+        if (!S.aborted) {
+          int slot;
+          const uint32_t md = MemoProbe(S.memo, C, &slot);
+          MemoStore(S.memo, C, slot, md, static_cast<uint32_t>(k + 1), 0, 0);
+        }
+        ret = false;
+        --sp;
+        continue;
+      }
+      const int v = PickRoot(S.use_scores, S.score, S.myadj, C, cand);
+      if (lane == 0) {
+        F->cand = cand & ~(1ull << v);
+        F->v = v;
+        F->pending = C & ~(1ull << v);
+        F->state = kNextComp;
+      }
+      Sync();
+      continue;
+    }
+    if (state == kNextComp) {
+      const uint64_t pending = Uniform64(F->pending);
+      if (pending == 0) {
+        const int v = Uniform32(F->v);
+        int slot;
+        const uint32_t md = MemoProbe(S.memo, C, &slot);
+        MemoStore(S.memo, C, slot, md, 0, static_cast<uint32_t>(k),
+                  static_cast<uint32_t>(v));
+        ret = true;
+        --sp;
+        continue;
+      }
+      // every component of C - v still unverified: one with a proven bound
+      // above k-1 rejects v now; components of at most k-1 vertices pass; the
+      // largest remaining one is searched next
+      uint64_t rest = pending, D = 0, trivial = 0;
+      int bestn = 0;
+      bool reject = false;
+      while (rest) {
+        const uint64_t Dc = Component(S.myadj, rest, rest & (0ull - rest), nullptr, nullptr);
+        rest &= ~Dc;
+        const int nd = __builtin_popcountll(Dc);
+        if (nd <= k - 1) {
+          trivial |= Dc;
+          continue;
+        }
+        const bool in_d = (Dc >> lane) & 1ull;
+        const int dd = in_d ? __builtin_popcountll(S.myadj & Dc) : 0;
+        const int ed = static_cast<int>(WaveSum(static_cast<uint32_t>(dd))) / 2;
+        if (EdgeLB(nd, ed) > k - 1) {
+          reject = true;
+          break;
+        }
+        if (nd > bestn) {
+          bestn = nd;
+          D = Dc;
+        }
+      }
+      if (reject) {
+        if (lane == 0) F->state = kNextRoot;
+        Sync();
+        continue;
+      }
+      if (!D) {  // only trivial components left
+        if (lane == 0) F->pending = 0;
+        Sync();
+        continue;
+      }
+      if (lane == 0) {
+        F->pending = pending & ~trivial;
+        F->curD = D;
+        F->state = kAfterChild;
+      }
+      Sync();
+      if (sp >= kStack) {  // cannot happen: depth <= 2 * 64 + 1
+        S.aborted = true;
+        ret = false;
+        --sp;
+        continue;
+      }
+      push(D, k - 1);
+      continue;
+    }
+    // kAfterChild
+    if (lane == 0) {
+      if (ret) {
+        F->pending &= ~F->curD;
+        F->state = kNextComp;
+      } else {
+        F->state = kNextRoot;
+      }
+    }
+    Sync();
+  }
+  return ret;
+}
+
+// Rebuild a tree of depth <= k for connected C from the memo's roots,
+// re-running the search for a set whose entry was evicted.  parent[] in LDS.
+__device__ void Build(Search& S, uint64_t C0, int k0, int par0, int8_t* parent,
+                      uint64_t* bst, int* bk, int* bpar) {
+  const int lane = threadIdx.x & 63;
+  int sp = 0;
+  auto push = [&](uint64_t C, int k, int par) {
+    if (lane == 0) {
+      bst[sp] = C;
+      bk[sp] = k;
+      bpar[sp] = par;
+    }
+    ++sp;
+    Sync();
+  };
+  push(C0, k0, par0);
+  while (sp > 0) {
+    --sp;
+    const uint64_t C = Uniform64(bst[sp]);
+    const int k = Uniform32(bk[sp]);
+    const int par = Uniform32(bpar[sp]);
+    const int n = __builtin_popcountll(C);
+    if (n <= k) {  // chain in ascending id order
+      int prev = par;
+      for (uint64_t m = C; m; m &= m - 1) {
+        const int u = __builtin_ctzll(m);
+        if (lane == 0) parent[u] = static_cast<int8_t>(prev);
+        prev = u;
+      }
+      Sync();
+      continue;
+    }
+    int slot;
+    uint32_t md = MemoProbe(S.memo, C, &slot);
+    uint32_t sk = (slot >= 0) ? (md >> 8) & 0xFF : 0;
+    if (!(sk && sk <= static_cast<uint32_t>(k))) {
+      const uint64_t saved_budget = S.budget;
+      S.budget = 0;  // the set is known feasible: finish the search
+      Feasible(S, C, k);
+      S.budget = saved_budget;
+      md = MemoProbe(S.memo, C, &slot);
+      sk = (slot >= 0) ? (md >> 8) & 0xFF : 0;
+      if (!(sk && sk <= static_cast<uint32_t>(k))) {
+        // no tree can be rebuilt: report the call as uncertified and keep
+        // the previous best tree
+        S.aborted = true;
+        return;
+      }
+    }
+    const int v = static_cast<int>((md >> 16) & 0xFF);
+    if (lane == 0) parent[v] = static_cast<int8_t>(par);
+    Sync();
+    uint64_t rest = C & ~(1ull << v);
+    while (rest) {
+      const uint64_t D = Component(S.myadj, rest, rest & (0ull - rest), nullptr, nullptr);
+      rest &= ~D;
+      push(D, static_cast<int>(sk) - 1, v);
+    }
+  }
+}
+
+// Greedy top-down tree: root of each component = best key (degree inside
+// the component, or the learned score).  Returns its depth; parent in LDS.
+__device__ int GreedyTree(Search& S, uint64_t V, bool use_scores, int8_t* parent,
+                          uint64_t* bst, int* bk, int* bpar) {
+  const int lane = threadIdx.x & 63;
+  int sp = 0, depth = 0;
+  uint64_t rest = V;
+  while (rest) {
+    const uint64_t D = Component(S.myadj, rest, rest & (0ull - rest), nullptr, nullptr);
+    rest &= ~D;
+    if (lane == 0) { bst[sp] = D; bk[sp] = 1; bpar[sp] = -1; }
+    ++sp;
+  }
+  Sync();
+  while (sp > 0) {
+    --sp;
+    const uint64_t C = Uniform64(bst[sp]);
+    const int d = Uniform32(bk[sp]);
+    const int par = Uniform32(bpar[sp]);
+    depth = max(depth, d);
+    const int v = PickRoot(use_scores, S.score, S.myadj, C, C);
+    if (lane == 0) parent[v] = static_cast<int8_t>(par);
+    uint64_t r = C & ~(1ull << v);
+    while (r) {
+      const uint64_t D = Component(S.myadj, r, r & (0ull - r), nullptr, nullptr);
+      r &= ~D;
+      if (lane == 0) { bst[sp] = D; bk[sp] = d + 1; bpar[sp] = v; }
+      ++sp;
+    }
+    Sync();
+  }
+  return depth;
+}
+
+// Minimum-degree elimination ordering with fill-in; the elimination tree of
+// the filled graph (parent = the neighbour eliminated next) is a treedepth
+// decomposition.  Sequential over vertices, lanes hold the filled rows.
+__device__ int MinDegreeTree(uint64_t myadj, uint64_t V, int8_t* parent,
+                             int* depth_tmp) {
+  const int lane = threadIdx.x & 63;
+  uint64_t fill = myadj;
+  uint64_t remain = V;
+  // elimination index per vertex, kept by its lane
+  int idx = 1 << 20;
+  int step = 0;
+  while (remain) {
+    const bool in = (remain >> lane) & 1ull;
+    const int d = in ? __builtin_popcountll(fill & remain) : 1 << 20;
+    // min degree, ties lowest id
+    uint64_t key = in ? (static_cast<uint64_t>(d) << 8) | static_cast<uint64_t>(lane) : ~0ull;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint64_t o = __shfl_xor(key, off);
+      key = o < key ? o : key;
+    }
+    const int v = static_cast<int>(key & 0xFF);
+    const uint64_t nv =
+        Uniform64(static_cast<uint64_t>(__shfl(static_cast<unsigned long long>(fill), v))) &
+        remain & ~(1ull << v);
+    // make N(v) a clique among the remaining vertices
+    if ((nv >> lane) & 1ull) fill |= nv & ~(1ull << lane);
+    if (lane == v) idx = step;
+    remain &= ~(1ull << v);
+    ++step;
+  }
+  // parent(v) = neighbour (in the filled graph) with the smallest later
+  // index; a uniform loop so every lane takes part in each shuffle
+  const bool in = (V >> lane) & 1ull;
+  int p = -1, best = 1 << 21;
+  for (int u = 0; u < 64; ++u) {
+    const int iu = __shfl(idx, u);
+    if (in && ((fill >> u) & 1ull) && ((V >> u) & 1ull) && iu > idx && iu < best) {
+      best = iu;
+      p = u;
+    }
+  }
+  if (in) parent[lane] = static_cast<int8_t>(p);
+  Sync();
+  // depth: iterate parent pointers (depth_tmp in LDS)
+  int dep = 0;
+  if (in) {
+    int u = lane;
+    dep = 1;
+    while (parent[u] >= 0 && dep <= 64) {
+      u = parent[u];
+      ++dep;
+    }
+  }
+  (void)depth_tmp;
+  uint32_t mx = in ? static_cast<uint32_t>(dep) : 0u;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = max(mx, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mx), off)));
+  return static_cast<int>(mx);
+}
+
+__global__ __launch_bounds__(64) void BnBKernel(
+    const BnBGraph* __restrict__ gs, BnBOut* __restrict__ outs,
+    uint4* __restrict__ memo_base, int memo_log2, uint64_t node_budget,
+    int ngraphs, uint32_t* __restrict__ queue, uint32_t tag_base) {
+  __shared__ Frame s_stack[kStack];
+  __shared__ int8_t s_parent[64], s_best[64], s_tmp[64];
+  __shared__ uint64_t s_bst[130];
+  __shared__ int s_bk[130], s_bpar[130];
+  __shared__ int s_gi;
+  __shared__ uint64_t s_comps[64];
+  const int lane = threadIdx.x & 63;
+  Search S;
+  S.st = s_stack;
+  S.memo.table = memo_base + (static_cast<size_t>(blockIdx.x) << (memo_log2 + 2));
+  S.memo.mask = (1u << memo_log2) - 1u;
+  for (;;) {
+    if (lane == 0) s_gi = static_cast<int>(atomicAdd(queue, 1u));
+    Sync();
+    const int gi = Uniform32(s_gi);
+    if (gi >= ngraphs) break;
+    const BnBGraph* g = gs + gi;
+    const int n = static_cast<int>(g->n);
+    const uint64_t V = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
+    S.myadj = lane < n ? g->adj[lane] : 0ull;
+    S.score = lane < n ? g->score[lane] : 0.f;
+    S.use_scores = g->use_scores != 0;
+    S.memo.tag = tag_base + static_cast<uint32_t>(gi) + 1u;
+    S.nodes = 0;
+    S.hits = 0;
+    S.budget = node_budget;
+    S.aborted = false;
+    // ---- initial upper bound: best of the greedy trees --------------------
+    int ub = GreedyTree(S, V, false, s_best, s_bst, s_bk, s_bpar);
+    if (S.use_scores) {
+      const int t = GreedyTree(S, V, true, s_parent, s_bst, s_bk, s_bpar);
+      if (t < ub) {
+        ub = t;
+        if (lane < 64) s_best[lane] = s_parent[lane];
+      }
+    }
+    {
+      const int t = MinDegreeTree(S.myadj, V, s_parent, nullptr);
+      if (t < ub) {
+        ub = t;
+        s_best[lane] = s_parent[lane];
+      }
+    }
+    Sync();
+    const int heur = ub;
+    // components of the input
+    uint64_t* comps = s_comps;
+    int ncomp = 0;
+    {
+      uint64_t rest = V;
+      while (rest) {
+        const uint64_t D = Component(S.myadj, rest, rest & (0ull - rest), nullptr, nullptr);
+        if (lane == 0) comps[ncomp] = D;
+        ++ncomp;
+        rest &= ~D;
+      }
+      Sync();
+    }
+    // caller's seed: accept it only with a tree of that depth
+    const int seed = g->ub_seed;
+    if (seed > 0 && seed < ub) {
+      bool ok = true;
+      for (int c = 0; c < ncomp && ok; ++c) ok = Feasible(S, Uniform64(comps[c]), seed);
+      if (ok && !S.aborted) {
+        s_parent[lane] = -1;
+        Sync();
+        for (int c = 0; c < ncomp && !S.aborted; ++c)
+          Build(S, Uniform64(comps[c]), seed, -1, s_parent, s_bst, s_bk, s_bpar);
+        if (!S.aborted) {
+          ub = seed;
+          s_best[lane] = s_parent[lane];
+          Sync();
+        }
+      }
+    }
+    // static lower bound of the whole graph
+    int lb = 1;
+    for (int c = 0; c < ncomp; ++c) {
+      const uint64_t Cc = Uniform64(comps[c]);
+      const bool in_c = (Cc >> lane) & 1ull;
+      lb = max(lb, LowerBound(S.myadj, in_c, Cc, 1 << 10));
+    }
+    lb = max(lb, Degeneracy(S.myadj, V) + 1);
+    // ---- descend: try ub-1 until it fails --------------------------------
+    while (ub - 1 >= lb && !S.aborted) {
+      const int k = ub - 1;
+      bool ok = true;
+      for (int c = 0; c < ncomp && ok; ++c) ok = Feasible(S, Uniform64(comps[c]), k);
+      if (S.aborted) break;
+      if (!ok) {
+        lb = ub;
+        break;
+      }
+      s_parent[lane] = -1;
+      Sync();
+      for (int c = 0; c < ncomp && !S.aborted; ++c)
+        Build(S, Uniform64(comps[c]), k, -1, s_parent, s_bst, s_bk, s_bpar);
+      if (S.aborted) break;
+      ub = k;
+      s_best[lane] = s_parent[lane];
+      Sync();
+    }
+    if (ub - 1 < lb) lb = ub;
+    BnBOut* o = outs + gi;
+    if (lane < n) o->parent[lane] = s_best[lane];
+    if (lane == 0) {
+      o->td = ub;
+      o->lb = lb;
+      o->certified = (!S.aborted && lb == ub) ? 1u : 0u;
+      o->heur_td = static_cast<uint32_t>(heur);
+      o->nodes = S.nodes;
+      o->memo_hits = S.hits;
+    }
+    Sync();
+  }
+}
+
+__global__ void BnBResetQueue(uint32_t* q) {
+  if (threadIdx.x == 0) *q = 0u;
+}
+
+template <typename F>
+int Guard(td_ctx* ctx, F&& f) {
+  if (!ctx) return TD_E_INVALID;
+  try {
+    TD_HIP(hipSetDevice(ctx->device));
+    f();
+    ctx->err.clear();
+    return TD_OK;
+  } catch (const Error& e) {
+    ctx->err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return TD_E_INVALID;
+  }
+}
+
+int64_t Param(td_ctx* ctx, const char* k, int64_t dflt) {
+  auto it = ctx->params.find(k);
+  return it == ctx->params.end() ? dflt : it->second;
+}
+
+void SolveBnB(td_ctx* ctx, int ngraphs, const int32_t* ns, const uint64_t* adj_concat,
+              const int32_t* ub_seed, const float* scores, uint8_t* td_out,
+              int8_t* parent_concat, uint8_t* certified_out, td_bnb_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  TD_REQUIRE(ngraphs >= 1 && ns && adj_concat, TD_E_INVALID, "empty batch or null pointer");
+  std::vector<size_t> off(ngraphs + 1, 0);
+  for (int i = 0; i < ngraphs; ++i) {
+    TD_REQUIRE(ns[i] >= 1 && ns[i] <= kMaxBnBN, TD_E_INVALID, "n must be in [1, 64]");
+    off[i + 1] = off[i] + ns[i];
+  }
+  std::vector<BnBGraph> hg(ngraphs);
+  for (int i = 0; i < ngraphs; ++i) {
+    ValidateGraph(ns[i], adj_concat + off[i], kMaxBnBN);
+    BnBGraph& G = hg[i];
+    std::memset(&G, 0, sizeof(G));
+    G.n = ns[i];
+    G.ub_seed = ub_seed ? ub_seed[i] : 0;
+    G.use_scores = scores ? 1u : 0u;
+    for (int v = 0; v < ns[i]; ++v) {
+      G.adj[v] = adj_concat[off[i] + v];
+      G.score[v] = scores ? scores[off[i] + v] : 0.f;
+    }
+  }
+  const int memo_log2 = static_cast<int>(Param(ctx, "bnb.memo_log2", 19));
+  TD_REQUIRE(memo_log2 >= 8 && memo_log2 <= 24, TD_E_INVALID, "bnb.memo_log2 in [8, 24]");
+  const uint64_t budget = static_cast<uint64_t>(Param(ctx, "bnb.node_budget", 2000000));
+  const int waves_cap = static_cast<int>(
+      Param(ctx, "bnb.waves", static_cast<int64_t>(ctx->prop.multiProcessorCount) * 4));
+  const int waves = std::max(1, std::min(ngraphs, waves_cap));
+  const size_t memo_bytes = (static_cast<size_t>(waves) << (memo_log2 + 2)) * sizeof(uint4);
+  auto* dG = static_cast<BnBGraph*>(ctx->Buffer("bnb.graphs", sizeof(BnBGraph) * ngraphs));
+  auto* dO = static_cast<BnBOut*>(ctx->Buffer("bnb.out", sizeof(BnBOut) * ngraphs));
+  auto* dM = static_cast<uint4*>(ctx->Buffer("bnb.memo", memo_bytes));
+  auto* dQ = static_cast<uint32_t*>(ctx->Buffer("bnb.queue", 256));
+  // Entries carry the tag of the graph that wrote them (a serial that grows
+  // across calls), so a table is cleared only when it is (re)allocated.
+  if (ctx->bnb_memo_ptr != dM || ctx->bnb_memo_bytes != memo_bytes ||
+      ctx->bnb_tag > 0xF0000000u - static_cast<uint32_t>(ngraphs)) {
+    TD_HIP(hipMemsetAsync(dM, 0, memo_bytes, ctx->stream));
+    ctx->bnb_memo_ptr = dM;
+    ctx->bnb_memo_bytes = memo_bytes;
+    ctx->bnb_tag = 0;
+  }
+  const uint32_t tag_base = ctx->bnb_tag;
+  ctx->bnb_tag += static_cast<uint32_t>(ngraphs);
+  TD_HIP(hipMemcpyAsync(dG, hg.data(), sizeof(BnBGraph) * ngraphs, hipMemcpyHostToDevice,
+                        ctx->stream));
+  TD_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+  hipLaunchKernelGGL(BnBResetQueue, dim3(1), dim3(64), 0, ctx->stream, dQ);
+  hipLaunchKernelGGL(BnBKernel, dim3(waves), dim3(64), 0, ctx->stream, dG, dO, dM,
+                     memo_log2, budget, ngraphs, dQ, tag_base);
+  TD_HIP(hipGetLastError());
+  TD_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+  std::vector<BnBOut> ho(ngraphs);
+  TD_HIP(hipMemcpyAsync(ho.data(), dO, sizeof(BnBOut) * ngraphs, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  TD_HIP(hipStreamSynchronize(ctx->stream));
+  uint64_t cert = 0, nodes = 0, hits = 0;
+  for (int i = 0; i < ngraphs; ++i) {
+    if (td_out) td_out[i] = static_cast<uint8_t>(ho[i].td);
+    if (certified_out) certified_out[i] = static_cast<uint8_t>(ho[i].certified);
+    if (parent_concat) std::memcpy(parent_concat + off[i], ho[i].parent, ns[i]);
+    cert += ho[i].certified;
+    nodes += ho[i].nodes;
+    hits += ho[i].memo_hits;
+  }
+  if (stats) {
+    float ms = 0.f;
+    TD_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    stats->graphs = static_cast<uint64_t>(ngraphs);
+    stats->certified = cert;
+    stats->nodes = nodes;
+    stats->memo_hits = hits;
+    stats->kernel_ms = ms;
+    stats->total_ms = std::chrono::duration<double, std::milli>(
+                          std::chrono::steady_clock::now() - t0)
+                          .count();
+  }
+}
+
+}  // namespace
+}  // namespace tdhip
+
+using namespace tdhip;  // NOLINT
+
+extern "C" {
+
+int td_bnb_solve_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
+                       const uint64_t* adj_concat, const int32_t* ub_seed,
+                       const float* vertex_scores, uint8_t* td_out,
+                       int8_t* parent_concat, uint8_t* certified_out,
+                       td_bnb_stats* stats) {
+  return Guard(ctx, [&] {
+    SolveBnB(ctx, ngraphs, n, adj_concat, ub_seed, vertex_scores, td_out, parent_concat,
+             certified_out, stats);
+  });
+}
+
+int td_bnb_solve(td_ctx* ctx, int32_t n, const uint64_t* adj, int32_t heur_td,
+                 int32_t* treedepth_out, int8_t* parent_out, uint8_t* certified_out) {
+  return Guard(ctx, [&] {
+    uint8_t td = 0, cert = 0;
+    const int32_t seed = heur_td;
+    SolveBnB(ctx, 1, &n, adj, &seed, nullptr, &td, parent_out, &cert, nullptr);
+    if (treedepth_out) *treedepth_out = td;
+    if (certified_out) *certified_out = cert;
+  });
+}
+
+}  // extern "C"

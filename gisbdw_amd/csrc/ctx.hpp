@@ -79,6 +79,11 @@ struct td_ctx {
 
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 
+  // branch-and-bound memo tables: entries are tagged per graph serial
+  void* bnb_memo_ptr = nullptr;
+  size_t bnb_memo_bytes = 0;
+  uint32_t bnb_tag = 0;
+
   // state of the last td_dp_solve (for the queries)
   int last_n = 0;
   bool have_dp = false;

@@ -107,6 +107,42 @@ int td_dp_solve_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
 int td_dp_autotune(td_ctx* ctx, int32_t n, const char* cache_path,
                    double* best_ms_out);
 
+/* ---------------------------------------------------------------------- */
+/* Branch-and-bound                                                        */
+/* ---------------------------------------------------------------------- */
+typedef struct td_bnb_stats {
+  uint64_t graphs;     /* graphs in the call                              */
+  uint64_t certified;  /* graphs whose treedepth was proven (LB = UB)     */
+  uint64_t nodes;      /* search nodes expanded, all graphs               */
+  uint64_t memo_hits;  /* nodes answered by the memo                      */
+  double kernel_ms;    /* GPU time of the search kernel (HIP events)      */
+  double total_ms;     /* host wall time of the whole call                */
+} td_bnb_stats;
+
+/* BnBGPU::operator()(g, heur_td) (Treedepth/src/bnb_gpu/bnb_gpu.hpp:13-31,
+ * bnb_gpu.cu:264-380): exact treedepth by branch-and-bound, n <= 64.
+ * heur_td is the caller's upper bound (the reference seeds best_td with it);
+ * it is used only if a tree of that depth is found, so a wrong seed cannot
+ * produce a wrong answer.  Unlike the reference, a decomposition is returned
+ * (parent array) and certified_out = 1 iff the search completed (the search
+ * stops after the "bnb.node_budget" parameter's node count, default 2e6). */
+int td_bnb_solve(td_ctx* ctx, int32_t n, const uint64_t* adj, int32_t heur_td,
+                 int32_t* treedepth_out, int8_t* parent_out,
+                 uint8_t* certified_out);
+
+/* Many graphs in one persistent launch, one wavefront per graph taken from a
+ * device queue (no reference counterpart; the reference solves one graph
+ * per process run).  ub_seed (nullable): per-graph upper bounds like
+ * heur_td.  vertex_scores (nullable, sum n[i] floats): a learned proposal;
+ * higher score = tried first as a root and used for an extra greedy upper
+ * bound.  Scores only order the search; they never prune.
+ * td_out / certified_out: ngraphs bytes; parent_concat: sum n[i] bytes. */
+int td_bnb_solve_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
+                       const uint64_t* adj_concat, const int32_t* ub_seed,
+                       const float* vertex_scores, uint8_t* td_out,
+                       int8_t* parent_concat, uint8_t* certified_out,
+                       td_bnb_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,109 @@
+"""GPU branch-and-bound (td_bnb_* in include/tdhip.h) against the oracle.
+
+Treedepth bit-exact against the subset DP (n <= 30) and the oracle's exact
+memo search (n in [40, 64] where it finishes); every returned tree passes the
+certifier with depth = td; certified only when the search completed."""
+import random
+
+import pytest
+
+from gisbdw_amd import api
+from gisbdw_amd import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+
+def small_graphs():
+    rng = random.Random(77)
+    gs = [G.path(n) for n in (1, 2, 3, 7, 12, 21)] + [G.cycle(n) for n in (3, 8, 13, 16)]
+    gs += [G.complete(n) for n in (1, 4, 8)] + [G.star(9), G.complete_bipartite(3, 5)]
+    gs += [G.chordal_cycle(n) for n in (10, 15)] + [G.edgeless(4)]
+    gs += [G.gnp(rng.randint(5, 20), rng.uniform(0.1, 0.3), rng.getrandbits(32)) for _ in range(40)]
+    gs += [G.grid_dropped(4, 5, 0.1, s) for s in range(3)] + [G.grid(5, 5), G.grid(4, 6)]
+    gs += [G.random_cubic(n, s) for n, s in ((16, 1), (20, 2), (24, 3))]
+    gs += [G.tree_plus_edges(n, 0.1, s) for n, s in ((20, 1), (28, 2), (30, 3))]
+    return gs
+
+
+def test_bnb_equals_dp_small(ctx, oracle):
+    gs = small_graphs()
+    want = [oracle.dp_fast(g)[0] for g in gs]
+    print("bnb small batch", len(gs), flush=True)
+    tds, pars, cert, st = api.bnb_solve_batch(ctx, gs)
+    for g, w, t, p, c in zip(gs, want, tds, pars, cert):
+        assert c, g.name
+        assert t == w, (g.name, t, w)
+        assert oracle.certify(g, p, t), g.name
+    assert st.certified == len(gs)
+
+
+def test_bnb_single_entry_and_seed(ctx, oracle):
+    b = api.BnBGPU(ctx)
+    g = G.gnp_connected(14, 0.3, 3)
+    td = oracle.dp_fast(g)[0]
+    assert b(g, 0) == td and b.certified
+    assert oracle.certify(g, b.parent, td)
+    # a seed above td, equal to td, and a wrong seed below td
+    for seed in (td + 2, td, td - 1, 1):
+        assert b(g, seed) == td and b.certified
+        assert oracle.certify(g, b.parent, td)
+    r, cert = b.solve(G.path(3))
+    assert (r.treedepth, r.root, cert) == (2, 1, True)
+
+
+def test_bnb_budget_gives_uncertified_valid_tree(ctx, oracle):
+    ctx.set_param("bnb.node_budget", 3)
+    try:
+        gs = [G.random_cubic(26, 5), G.grid(5, 6)]
+        tds, pars, cert, st = api.bnb_solve_batch(ctx, gs)
+        for g, t, p, c in zip(gs, tds, pars, cert):
+            assert oracle.certify(g, p, t)
+            assert t >= oracle.dp_fast(g)[0]
+        assert not all(cert)
+    finally:
+        ctx.set_param("bnb.node_budget", 2000000)
+
+
+def test_bnb_large_trees_vs_exact(ctx, oracle):
+    gs = [G.tree_plus_edges(n, 0.1, s) for n, s in ((40, 11), (48, 12), (56, 13), (64, 14))]
+    print("bnb large", flush=True)
+    tds, pars, cert, st = api.bnb_solve_batch(ctx, gs)
+    for g, t, p, c in zip(gs, tds, pars, cert):
+        assert oracle.certify(g, p, t)
+        if c:
+            et, ep, nodes = oracle.exact(g, node_budget=200000)
+            if et > 0:
+                assert et == t, g.name
+
+
+def test_bnb_scores_do_not_change_answers(ctx, oracle):
+    rng = random.Random(9)
+    gs = [G.gnp(rng.randint(8, 20), rng.uniform(0.15, 0.5), rng.getrandbits(32)) for _ in range(24)]
+    want = [oracle.dp_fast(g)[0] for g in gs]
+    scores = [[rng.uniform(-3, 3) for _ in range(g.n)] for g in gs]
+    tds, pars, cert, st = api.bnb_solve_batch(ctx, gs, scores=scores)
+    for g, w, t, p, c in zip(gs, want, tds, pars, cert):
+        assert c and t == w and oracle.certify(g, p, t)
+
+
+def test_bnb_disconnected_and_errors(ctx, oracle):
+    a, b = G.path(7), G.cycle(9)
+    g = G.Graph(16, a.adj + [r << 7 for r in b.adj])
+    tds, pars, cert, st = api.bnb_solve_batch(ctx, [g])
+    assert cert[0] and tds[0] == oracle.dp_fast(g)[0] and oracle.certify(g, pars[0], tds[0])
+    with pytest.raises(api.TDError):
+        api.bnb_solve_batch(ctx, [G.Graph(2, [0b01, 0])])
+
+
+def test_bnb_dense_exact_when_certified(ctx, oracle):
+    """Dense graphs can exhaust the node budget; whatever is returned is a
+    valid tree, and a certified answer equals the DP."""
+    rng = random.Random(78)
+    gs = [G.gnp(rng.randint(10, 20), rng.uniform(0.3, 0.7), rng.getrandbits(32)) for _ in range(16)]
+    want = [oracle.dp_fast(g)[0] for g in gs]
+    tds, pars, cert, st = api.bnb_solve_batch(ctx, gs)
+    for g, w, t, p, c in zip(gs, want, tds, pars, cert):
+        assert oracle.certify(g, p, t)
+        assert t >= w
+        if c:
+            assert t == w, g.name
