@@ -105,6 +105,23 @@ def load_library():
         "td_bnb_solve": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_uint64), C.c_int32,
                                    P(C.c_int32), P(C.c_int8), P(C.c_uint8)]),
     })
+    sig.update({
+        "td_mlp_num_params": (C.c_int, []),
+        "td_mlp_init": (C.c_int, [C.c_void_p, C.c_uint64]),
+        "td_mlp_set_weights": (C.c_int, [C.c_void_p, P(C.c_float), C.c_int64]),
+        "td_mlp_get_weights": (C.c_int, [C.c_void_p, P(C.c_float), C.c_int64]),
+        "td_mlp_features": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
+                                      P(C.c_float)]),
+        "td_mlp_scores": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
+                                    P(C.c_float)]),
+        "td_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
+                                       P(C.c_uint64), P(C.c_float), P(C.c_float)]),
+        "td_mlp_train": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
+                                   P(C.c_uint64), C.c_int32, C.c_int32, C.c_float, C.c_uint64,
+                                   P(C.c_float)]),
+        "td_mlp_make_samples": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, P(C.c_int32),
+                                          P(C.c_uint64), P(C.c_uint64), P(C.c_uint8)]),
+    })
     optional = {}
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

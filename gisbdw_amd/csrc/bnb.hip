@@ -609,7 +609,7 @@ __global__ __launch_bounds__(64) void BnBKernel(
     uint4* __restrict__ memo_base, int memo_log2, uint64_t node_budget,
     int ngraphs, uint32_t* __restrict__ queue, uint32_t tag_base) {
   __shared__ Frame s_stack[kStack];
-  __shared__ int8_t s_parent[64], s_best[64], s_tmp[64];
+  __shared__ int8_t s_parent[64], s_best[64];
   __shared__ uint64_t s_bst[130];
   __shared__ int s_bk[130], s_bpar[130];
   __shared__ int s_gi;

@@ -1,0 +1,840 @@
+// mlp.hip -- the learned proposal: a message-passing MLP that scores
+// vertices as roots of an elimination tree.  BF16 weights and activations on
+// MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation; hand-written
+// forward and backward; Adam on FP32 master weights.
+//
+// It has no counterpart in the reference; it plugs into the reference's two
+// extension points of BranchAndBound: LowerBound::LowerBoundInfo::attempt_order
+// (Treedepth/src/branch_and_bound/branch_and_bound.hpp:21-24, used at
+// branch_and_bound.cpp:81-87) and Heuristic::Get (branch_and_bound.hpp:38-42,
+// used at :78).  It only orders the search and seeds its upper bound.
+//
+// Network (n <= 64 vertices, rows padded to 64, H = 64):
+//   x_v   = [deg/4, core/4, twohop/16, ecc/8]
+//   h0    = relu(W0 x + b0)                                   (4 -> 64)
+//   m_r   = mean_{u in N(v)} h_{r-1,u}                        (r = 1, 2)
+//   h_r   = relu(Ws_r h_{r-1} + Wn_r m_r + b_r)               (64+64 -> 64)
+//   z1    = relu(W3 h2 + b3), z2 = relu(W4 z1 + b4)           (64 -> 64)
+//   s_v   = w5 . z2 + b5                                      (64 -> 1)
+// Loss per graph: cross-entropy of softmax(s) over its vertices against the
+// uniform distribution on the optimal roots.  Values are rounded to bf16 at
+// every MFMA input; the CPU reference in tests/ rounds at the same points.
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace tdhip {
+namespace {
+
+constexpr int kH = 64;
+// parameter offsets in the FP32 master vector
+constexpr int P_W0 = 0, P_B0 = 256, P_WS1 = 320, P_WN1 = 4416, P_B1 = 8512,
+              P_WS2 = 8576, P_WN2 = 12672, P_B2 = 16768, P_W3 = 16832,
+              P_B3 = 20928, P_W4 = 20992, P_B4 = 25088, P_W5 = 25152,
+              P_B5 = 25216, kNParams = 25217;
+// per-graph scratch: 7 activation matrices + 4 gradient matrices (bf16,
+// 64x64 each) + 2 fp32 matrices
+constexpr int kMat = kH * kH;
+enum { A_H0 = 0, A_M1, A_H1, A_M2, A_H2, A_Z1, A_Z2, G_A, G_B, G_M, G_T, kNBf16 };
+constexpr size_t kScratchBf16 = static_cast<size_t>(kNBf16) * kMat;  // uint16
+constexpr size_t kScratchF32 = 2 * kMat;
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline uint16_t ToBf16(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return static_cast<uint16_t>(u >> 16);
+}
+
+__host__ __device__ inline float FromBf16(uint16_t h) {
+  return __builtin_bit_cast(float, static_cast<uint32_t>(h) << 16);
+}
+
+__device__ __forceinline__ void Sync() { __syncthreads(); }
+
+// acc[tm][tn][i] holds D[16 tm + 4 (lane/16) + i][16 tn + lane%16].
+struct Acc {
+  v4f t[4][4];
+};
+
+__device__ __forceinline__ void Zero(Acc& a) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a.t[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+}
+
+// acc += A (64xK=64) * B (64x64); A(m,k) = A[m*am + k*ak], B(k,n) =
+// B[k*bk + n*bn], bf16 bit patterns.  Lane l supplies row/col l%16 and the
+// k-group l/16 (8 consecutive k) of each 16x16x32 step.
+__device__ __forceinline__ void MatMul(Acc& acc, const uint16_t* __restrict__ A,
+                                       int am, int ak,
+                                       const uint16_t* __restrict__ B, int bk,
+                                       int bn) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) {
+    v8s a[4], b[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = 32 * ks + 8 * g + j;
+        a[t][j] = static_cast<short>(A[(16 * t + r) * am + k * ak]);
+        b[t][j] = static_cast<short>(B[k * bk + (16 * t + r) * bn]);
+      }
+    }
+#pragma unroll
+    for (int tm = 0; tm < 4; ++tm)
+#pragma unroll
+      for (int tn = 0; tn < 4; ++tn)
+        acc.t[tm][tn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            __builtin_bit_cast(v8bf, a[tm]), __builtin_bit_cast(v8bf, b[tn]),
+            acc.t[tm][tn], 0, 0, 0);
+  }
+}
+
+// element (row, col) owned by this lane for tile (tm, tn), register i
+__device__ __forceinline__ int AccRow(int tm, int i) {
+  return 16 * tm + 4 * ((threadIdx.x & 63) >> 4) + i;
+}
+__device__ __forceinline__ int AccCol(int tn) { return 16 * tn + (threadIdx.x & 15); }
+
+// Y = relu(acc + bias[col]) -> bf16 at Y[row*64 + col]
+__device__ __forceinline__ void StoreReluBf16(const Acc& acc, const float* bias,
+                                              uint16_t* Y) {
+#pragma unroll
+  for (int tm = 0; tm < 4; ++tm)
+#pragma unroll
+    for (int tn = 0; tn < 4; ++tn)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = AccRow(tm, i), col = AccCol(tn);
+        const float v = acc.t[tm][tn][i] + bias[col];
+        Y[row * kH + col] = ToBf16(v > 0.f ? v : 0.f);
+      }
+}
+
+__device__ __forceinline__ void StoreF32(const Acc& acc, float* Y, int ym, int yn) {
+#pragma unroll
+  for (int tm = 0; tm < 4; ++tm)
+#pragma unroll
+    for (int tn = 0; tn < 4; ++tn)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        Y[AccRow(tm, i) * ym + AccCol(tn) * yn] = acc.t[tm][tn][i];
+}
+
+// m[v] = mean over neighbours u of h[u] (fp32 sum of bf16 values), rounded
+__device__ __forceinline__ void Aggregate(const uint16_t* h, uint16_t* m,
+                                          uint64_t myadj) {
+  const int v = threadIdx.x & 63;
+  const int deg = __builtin_popcountll(myadj);
+  const float inv = deg ? 1.0f / static_cast<float>(deg) : 0.f;
+  for (int c = 0; c < kH; ++c) {
+    float s = 0.f;
+    for (uint64_t a = myadj; a; a &= a - 1) s += FromBf16(h[__builtin_ctzll(a) * kH + c]);
+    m[v * kH + c] = ToBf16(s * inv);
+  }
+}
+
+struct Net {
+  const float* w;      // fp32 master (biases, W0 and w5 are read as bf16 values)
+  const uint16_t* wb;  // bf16 copy
+};
+
+__device__ __forceinline__ float Wb(const Net& N, int p) { return FromBf16(N.wb[p]); }
+
+// Forward for one graph (this wave); activations to scratch; returns s_v in
+// lane v.
+__device__ float Forward(const Net& N, const float* __restrict__ x, uint64_t myadj,
+                         uint16_t* S) {
+  const int v = threadIdx.x & 63;
+  uint16_t* h0 = S + A_H0 * kMat;
+  uint16_t* m1 = S + A_M1 * kMat;
+  uint16_t* h1 = S + A_H1 * kMat;
+  uint16_t* m2 = S + A_M2 * kMat;
+  uint16_t* h2 = S + A_H2 * kMat;
+  uint16_t* z1 = S + A_Z1 * kMat;
+  uint16_t* z2 = S + A_Z2 * kMat;
+  const float xv[4] = {x[v * 4 + 0], x[v * 4 + 1], x[v * 4 + 2], x[v * 4 + 3]};
+  for (int o = 0; o < kH; ++o) {
+    float a = N.w[P_B0 + o];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a += Wb(N, P_W0 + o * 4 + i) * xv[i];
+    h0[v * kH + o] = ToBf16(a > 0.f ? a : 0.f);
+  }
+  Sync();
+  Aggregate(h0, m1, myadj);
+  Sync();
+  Acc acc;
+  Zero(acc);
+  MatMul(acc, h0, kH, 1, N.wb + P_WS1, 1, kH);
+  MatMul(acc, m1, kH, 1, N.wb + P_WN1, 1, kH);
+  StoreReluBf16(acc, N.w + P_B1, h1);
+  Sync();
+  Aggregate(h1, m2, myadj);
+  Sync();
+  Zero(acc);
+  MatMul(acc, h1, kH, 1, N.wb + P_WS2, 1, kH);
+  MatMul(acc, m2, kH, 1, N.wb + P_WN2, 1, kH);
+  StoreReluBf16(acc, N.w + P_B2, h2);
+  Sync();
+  Zero(acc);
+  MatMul(acc, h2, kH, 1, N.wb + P_W3, 1, kH);
+  StoreReluBf16(acc, N.w + P_B3, z1);
+  Sync();
+  Zero(acc);
+  MatMul(acc, z1, kH, 1, N.wb + P_W4, 1, kH);
+  StoreReluBf16(acc, N.w + P_B4, z2);
+  Sync();
+  float s = N.w[P_B5];
+  for (int j = 0; j < kH; ++j) s += FromBf16(z2[v * kH + j]) * Wb(N, P_W5 + j);
+  return s;
+}
+
+__device__ __forceinline__ float WaveMaxF(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ __forceinline__ float WaveSumF(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// features: one wave per graph, lanes = vertices
+// ---------------------------------------------------------------------------
+__device__ void Features(uint64_t myadj, int n, float* x) {
+  __shared__ uint64_t s_adj[64];
+  const int v = threadIdx.x & 63;
+  s_adj[v] = myadj;
+  Sync();
+  const bool in = v < n;
+  const uint64_t V = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
+  const int deg = __builtin_popcountll(myadj);
+  // core number: largest k whose k-core holds v (parallel peeling)
+  int core = 0;
+  for (int k = 1; k < 64; ++k) {
+    uint64_t X = V;
+    for (;;) {
+      const bool inx = (X >> v) & 1ull;
+      const int d = inx ? __builtin_popcountll(myadj & X) : 0;
+      const uint64_t keep = static_cast<uint64_t>(__ballot(inx && d >= k));
+      if (keep == X) break;
+      X = keep;
+    }
+    if (!X) break;
+    if ((X >> v) & 1ull) core = k;
+  }
+  // vertices within distance 2, and BFS eccentricity inside v's component
+  uint64_t n2 = myadj;
+  for (uint64_t a = myadj; a; a &= a - 1) n2 |= s_adj[__builtin_ctzll(a)];
+  n2 &= ~(1ull << v);
+  int ecc = 0;
+  if (in) {
+    uint64_t seen = 1ull << v, front = seen;
+    for (;;) {
+      uint64_t nb = 0;
+      for (uint64_t f = front; f; f &= f - 1) nb |= s_adj[__builtin_ctzll(f)];
+      nb &= ~seen;
+      if (!nb) break;
+      seen |= nb;
+      front = nb;
+      ++ecc;
+    }
+  }
+  x[v * 4 + 0] = in ? deg * 0.25f : 0.f;
+  x[v * 4 + 1] = in ? core * 0.25f : 0.f;
+  x[v * 4 + 2] = in ? __builtin_popcountll(n2) * 0.0625f : 0.f;
+  x[v * 4 + 3] = in ? ecc * 0.125f : 0.f;
+  Sync();
+}
+
+__global__ __launch_bounds__(64) void MLPFeatureKernel(const int32_t* __restrict__ ns,
+                                                       const uint64_t* __restrict__ adj,
+                                                       float* __restrict__ feat) {
+  const int gi = blockIdx.x, v = threadIdx.x;
+  const int n = ns[gi];
+  const uint64_t myadj = v < n ? adj[static_cast<size_t>(gi) * 64 + v] : 0ull;
+  Features(myadj, n, feat + static_cast<size_t>(gi) * 256);
+}
+
+__global__ __launch_bounds__(64) void MLPScoreKernel(Net N, const int32_t* __restrict__ ns,
+                                                     const uint64_t* __restrict__ adj,
+                                                     const float* __restrict__ feat,
+                                                     uint16_t* __restrict__ scratch,
+                                                     float* __restrict__ scores) {
+  const int gi = blockIdx.x, v = threadIdx.x;
+  const int n = ns[gi];
+  const uint64_t myadj = v < n ? adj[static_cast<size_t>(gi) * 64 + v] : 0ull;
+  const float s = Forward(N, feat + static_cast<size_t>(gi) * 256, myadj,
+                          scratch + static_cast<size_t>(blockIdx.x) * kScratchBf16);
+  scores[static_cast<size_t>(gi) * 64 + v] = v < n ? s : 0.f;
+}
+
+// Forward + backward of one graph per wave.  Gradient of
+// scale * loss_i goes to part[i][*] (one row per graph, reduced later).
+__global__ __launch_bounds__(64) void MLPTrainKernel(
+    Net N, const int32_t* __restrict__ ns, const uint64_t* __restrict__ adj,
+    const uint64_t* __restrict__ targets, const float* __restrict__ feat,
+    const int32_t* __restrict__ index, uint16_t* __restrict__ scratch,
+    float* __restrict__ scratchf, float* __restrict__ part, float* __restrict__ loss,
+    float scale) {
+  const int bi = blockIdx.x;
+  const int gi = index ? index[bi] : bi;
+  const int v = threadIdx.x;
+  const int n = ns[gi];
+  const bool in = v < n;
+  const uint64_t myadj = in ? adj[static_cast<size_t>(gi) * 64 + v] : 0ull;
+  uint16_t* S = scratch + static_cast<size_t>(bi) * kScratchBf16;
+  float* SF = scratchf + static_cast<size_t>(bi) * kScratchF32;
+  float* G = part + static_cast<size_t>(bi) * kNParams;
+  const float* x = feat + static_cast<size_t>(gi) * 256;
+  const float s = Forward(N, x, myadj, S);
+  // ---- loss: softmax over the graph's vertices vs uniform optimal roots
+  const uint64_t tgt = targets[gi];
+  const int npos = __builtin_popcountll(tgt);
+  const float mx = WaveMaxF(in ? s : -INFINITY);
+  const float e = in ? expf(s - mx) : 0.f;
+  const float z = WaveSumF(e);
+  const float p = e / z;
+  const float t = ((tgt >> v) & 1ull) ? 1.0f / static_cast<float>(npos) : 0.f;
+  const float li = (t > 0.f) ? -t * (s - mx - logf(z)) : 0.f;
+  const float lsum = WaveSumF(li);
+  if (v == 0) loss[bi] = lsum;
+  const float ds = in ? (p - t) * scale : 0.f;  // dL/ds_v
+  uint16_t* h0 = S + A_H0 * kMat;
+  uint16_t* m1 = S + A_M1 * kMat;
+  uint16_t* h1 = S + A_H1 * kMat;
+  uint16_t* m2 = S + A_M2 * kMat;
+  uint16_t* h2 = S + A_H2 * kMat;
+  uint16_t* z1 = S + A_Z1 * kMat;
+  uint16_t* z2 = S + A_Z2 * kMat;
+  uint16_t* gA = S + G_A * kMat;  // bf16 gradient (pre-activation)
+  uint16_t* gM = S + G_M * kMat;  // bf16 gradient w.r.t. an aggregate
+  float* F0 = SF;                 // fp32 gradient w.r.t. an activation
+  float* F1 = SF + kMat;
+  // ---- head: s = w5 . z2 + b5
+  // dw5[j] = sum_r ds_r z2[r][j] (lane j); db5 = sum_r ds_r
+  {
+    float dw = 0.f;
+    for (int r = 0; r < kH; ++r) dw += __shfl(ds, r) * FromBf16(z2[r * kH + v]);
+    G[P_W5 + v] = dw;
+    const float db = WaveSumF(ds);
+    if (v == 0) G[P_B5] = db;
+    // da4[r][j] = ds_r w5[j] (z2 > 0)  (lane r)
+    for (int j = 0; j < kH; ++j) {
+      const float zz = FromBf16(z2[v * kH + j]);
+      gA[v * kH + j] = ToBf16(zz > 0.f ? ds * Wb(N, P_W5 + j) : 0.f);
+    }
+  }
+  Sync();
+  // dense layer backward: given gA = d pre-activation (bf16, [r][o]),
+  // input X ([r][i]) and weight W ([o][i]): dW = gA^T X, db = colsum gA,
+  // dX (fp32, [r][i]) = gA W into Fout.
+  auto dense_back = [&](const uint16_t* Xin, int pW, int pB, float* Fout) {
+    Acc acc;
+    Zero(acc);
+    MatMul(acc, gA, 1, kH, Xin, kH, 1);  // (o, r) x (r, i)
+    StoreF32(acc, G + pW, kH, 1);
+    float db = 0.f;
+    for (int r = 0; r < kH; ++r) db += FromBf16(gA[r * kH + v]);
+    G[pB + v] = db;
+    if (Fout) {
+      Zero(acc);
+      MatMul(acc, gA, kH, 1, N.wb + pW, kH, 1);  // (r, o) x (o, i)
+      StoreF32(acc, Fout, kH, 1);
+    }
+  };
+  // layer 4: z2 = relu(W4 z1 + b4)
+  dense_back(z1, P_W4, P_B4, F0);
+  Sync();
+  for (int j = 0; j < kH; ++j) {
+    const float y = FromBf16(z1[v * kH + j]);
+    gA[v * kH + j] = ToBf16(y > 0.f ? F0[v * kH + j] : 0.f);
+  }
+  Sync();
+  // layer 3: z1 = relu(W3 h2 + b3)
+  dense_back(h2, P_W3, P_B3, F0);
+  Sync();
+  for (int j = 0; j < kH; ++j) {
+    const float y = FromBf16(h2[v * kH + j]);
+    gA[v * kH + j] = ToBf16(y > 0.f ? F0[v * kH + j] : 0.f);
+  }
+  Sync();
+  // message-passing layer: h_out = relu(Ws h_in + Wn m + b), m = mean_N h_in
+  auto mp_back = [&](const uint16_t* hin, const uint16_t* mag, int pWs, int pWn, int pB) {
+    dense_back(hin, pWs, pB, F0);  // dWs, db, F0 = gA Ws
+    Acc acc;
+    Zero(acc);
+    MatMul(acc, gA, 1, kH, mag, kH, 1);
+    StoreF32(acc, G + pWn, kH, 1);
+    Zero(acc);
+    MatMul(acc, gA, kH, 1, N.wb + pWn, kH, 1);
+    StoreF32(acc, F1, kH, 1);  // dL/dm
+    Sync();
+    // d h_in[u] += sum_{v in N(u)} dm[v] / deg_v   (lane u)
+    for (int c = 0; c < kH; ++c) {
+      float a = F0[v * kH + c];
+      for (uint64_t nb = myadj; nb; nb &= nb - 1) {
+        const int w = __builtin_ctzll(nb);
+        const int dw = __builtin_popcountll(adj[static_cast<size_t>(gi) * 64 + w]);
+        a += F1[w * kH + c] / static_cast<float>(dw);
+      }
+      F0[v * kH + c] = a;
+    }
+    Sync();
+  };
+  mp_back(h1, m2, P_WS2, P_WN2, P_B2);
+  for (int j = 0; j < kH; ++j) {
+    const float y = FromBf16(h1[v * kH + j]);
+    gA[v * kH + j] = ToBf16(y > 0.f ? F0[v * kH + j] : 0.f);
+  }
+  Sync();
+  mp_back(h0, m1, P_WS1, P_WN1, P_B1);
+  // layer 0: h0 = relu(W0 x + b0), computed in fp32 from the fp32 d h0
+  {
+    // lane o accumulates dW0[o][0..3] and db0[o] over rows r
+    float dw[4] = {0.f, 0.f, 0.f, 0.f}, db = 0.f;
+    for (int r = 0; r < kH; ++r) {
+      const float y = FromBf16(h0[r * kH + v]);
+      const float da = y > 0.f ? F0[r * kH + v] : 0.f;
+      db += da;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) dw[i] += da * x[r * 4 + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) G[P_W0 + v * 4 + i] = dw[i];
+    G[P_B0 + v] = db;
+  }
+  (void)gM;
+}
+
+// grad[p] = sum over rows of part[*][p], fixed order (deterministic)
+__global__ void MLPReduceKernel(const float* __restrict__ part, int rows,
+                                float* __restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kNParams) return;
+  float s = 0.f;
+  for (int r = 0; r < rows; ++r) s += part[static_cast<size_t>(r) * kNParams + p];
+  grad[p] = s;
+}
+
+__global__ void MLPAdamKernel(float* __restrict__ w, uint16_t* __restrict__ wb,
+                              float* __restrict__ m1, float* __restrict__ m2,
+                              const float* __restrict__ g, float lr, float b1,
+                              float b2, float eps, float c1, float c2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kNParams) return;
+  const float gp = g[p];
+  const float a = b1 * m1[p] + (1.f - b1) * gp;
+  const float b = b2 * m2[p] + (1.f - b2) * gp * gp;
+  m1[p] = a;
+  m2[p] = b;
+  const float nw = w[p] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+  w[p] = nw;
+  wb[p] = ToBf16(nw);
+}
+
+__global__ void MLPToBf16Kernel(const float* __restrict__ w, uint16_t* __restrict__ wb) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < kNParams) wb[p] = ToBf16(w[p]);
+}
+
+// Optimal-root labels from the last single-graph DP: for a connected vertex
+// set S (one per wave), positives = {v in S : td(S \ v) = td(S) - 1}.
+struct DPLookup {
+  const uint8_t* arena;
+  uint32_t lb;
+  uint32_t lvl[34];
+};
+
+__device__ __forceinline__ uint32_t LookupTd(const DPLookup& L, const uint32_t* cb, uint32_t S) {
+  const uint32_t low = S & ((1u << L.lb) - 1u);
+  uint32_t H = S >> L.lb;
+  const int h = __builtin_popcount(H);
+  uint32_t r = 0;
+  int i = 0;
+  while (H) {
+    const int x = __builtin_ctz(H);
+    H &= H - 1;
+    r += cb[x * 33 + (++i)];
+  }
+  return static_cast<uint32_t>(L.arena[((static_cast<size_t>(L.lvl[h]) + r) << L.lb) + low]) >> 1;
+}
+
+__global__ __launch_bounds__(64) void LabelKernel(DPLookup L, const uint32_t* __restrict__ g_cb,
+                                                  const uint32_t* __restrict__ sets,
+                                                  uint32_t* __restrict__ pos,
+                                                  uint8_t* __restrict__ tdout) {
+  __shared__ uint32_t s_cb[32 * 33];
+  for (int i = threadIdx.x; i < 32 * 33; i += 64) s_cb[i] = g_cb[i];
+  Sync();
+  const uint32_t S = sets[blockIdx.x];
+  const int v = threadIdx.x;
+  const uint32_t T = LookupTd(L, s_cb, S);
+  const bool in = v < 32 && ((S >> v) & 1u);
+  const uint32_t tv = in ? LookupTd(L, s_cb, S & ~(1u << v)) : 0xFFu;
+  const uint64_t b = static_cast<uint64_t>(__ballot(in && tv + 1u == T));
+  if (v == 0) {
+    pos[blockIdx.x] = static_cast<uint32_t>(b);
+    tdout[blockIdx.x] = static_cast<uint8_t>(T);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+template <typename F>
+int Guard(td_ctx* ctx, F&& f) {
+  if (!ctx) return TD_E_INVALID;
+  try {
+    TD_HIP(hipSetDevice(ctx->device));
+    f();
+    ctx->err.clear();
+    return TD_OK;
+  } catch (const Error& e) {
+    ctx->err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return TD_E_INVALID;
+  }
+}
+
+struct DevNet {
+  float* w;
+  uint16_t* wb;
+  float* m1;
+  float* m2;
+};
+
+DevNet NetBuffers(td_ctx* ctx) {
+  DevNet d;
+  const bool fresh = ctx->bufs.find("mlp.w") == ctx->bufs.end();
+  d.w = static_cast<float*>(ctx->Buffer("mlp.w", kNParams * 4));
+  d.wb = static_cast<uint16_t*>(ctx->Buffer("mlp.wb", kNParams * 2));
+  d.m1 = static_cast<float*>(ctx->Buffer("mlp.adam1", kNParams * 4));
+  d.m2 = static_cast<float*>(ctx->Buffer("mlp.adam2", kNParams * 4));
+  if (fresh) {
+    TD_HIP(hipMemsetAsync(d.w, 0, kNParams * 4, ctx->stream));
+    TD_HIP(hipMemsetAsync(d.wb, 0, kNParams * 2, ctx->stream));
+    TD_HIP(hipMemsetAsync(d.m1, 0, kNParams * 4, ctx->stream));
+    TD_HIP(hipMemsetAsync(d.m2, 0, kNParams * 4, ctx->stream));
+  }
+  return d;
+}
+
+void CheckGraphs(int ngraphs, const int32_t* ns, const uint64_t* adj64) {
+  TD_REQUIRE(ngraphs >= 1 && ns && adj64, TD_E_INVALID, "empty batch or null pointer");
+  for (int i = 0; i < ngraphs; ++i) {
+    TD_REQUIRE(ns[i] >= 1 && ns[i] <= 64, TD_E_INVALID, "n must be in [1, 64]");
+    ValidateGraph(ns[i], adj64 + static_cast<size_t>(i) * 64, 64);
+    for (int v = ns[i]; v < 64; ++v)
+      TD_REQUIRE(adj64[static_cast<size_t>(i) * 64 + v] == 0, TD_E_INVALID,
+                 "rows beyond n must be zero");
+  }
+}
+
+struct Uploaded {
+  int32_t* ns;
+  uint64_t* adj;
+  float* feat;
+};
+
+Uploaded UploadGraphs(td_ctx* ctx, const char* tag, int ngraphs, const int32_t* ns,
+                      const uint64_t* adj64) {
+  Uploaded u;
+  const std::string t(tag);
+  u.ns = static_cast<int32_t*>(ctx->Buffer(t + ".n", 4ull * ngraphs));
+  u.adj = static_cast<uint64_t*>(ctx->Buffer(t + ".adj", 512ull * ngraphs));
+  u.feat = static_cast<float*>(ctx->Buffer(t + ".feat", 1024ull * ngraphs));
+  TD_HIP(hipMemcpyAsync(u.ns, ns, 4ull * ngraphs, hipMemcpyHostToDevice, ctx->stream));
+  TD_HIP(hipMemcpyAsync(u.adj, adj64, 512ull * ngraphs, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(MLPFeatureKernel, dim3(ngraphs), dim3(64), 0, ctx->stream, u.ns, u.adj,
+                     u.feat);
+  TD_HIP(hipGetLastError());
+  return u;
+}
+
+}  // namespace
+}  // namespace tdhip
+
+using namespace tdhip;  // NOLINT
+
+extern "C" {
+
+int td_mlp_num_params(void) { return kNParams; }
+
+int td_mlp_init(td_ctx* ctx, uint64_t seed) {
+  return Guard(ctx, [&] {
+    std::vector<float> w(kNParams, 0.f);
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<float> U(-1.f, 1.f);
+    auto fill = [&](int off, int rows, int cols) {
+      const float a = std::sqrt(6.0f / static_cast<float>(cols));
+      for (int i = 0; i < rows * cols; ++i) w[off + i] = a * U(rng);
+    };
+    fill(P_W0, 64, 4);
+    fill(P_WS1, 64, 64);
+    fill(P_WN1, 64, 64);
+    fill(P_WS2, 64, 64);
+    fill(P_WN2, 64, 64);
+    fill(P_W3, 64, 64);
+    fill(P_W4, 64, 64);
+    fill(P_W5, 1, 64);
+    DevNet d = NetBuffers(ctx);
+    TD_HIP(hipMemcpyAsync(d.w, w.data(), kNParams * 4, hipMemcpyHostToDevice, ctx->stream));
+    TD_HIP(hipMemsetAsync(d.m1, 0, kNParams * 4, ctx->stream));
+    TD_HIP(hipMemsetAsync(d.m2, 0, kNParams * 4, ctx->stream));
+    hipLaunchKernelGGL(MLPToBf16Kernel, dim3((kNParams + 255) / 256), dim3(256), 0,
+                       ctx->stream, d.w, d.wb);
+    TD_HIP(hipGetLastError());
+    ctx->params["mlp.adam_t"] = 0;
+    TD_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int td_mlp_set_weights(td_ctx* ctx, const float* w, int64_t count) {
+  return Guard(ctx, [&] {
+    TD_REQUIRE(w && count == kNParams, TD_E_INVALID, "need td_mlp_num_params() floats");
+    DevNet d = NetBuffers(ctx);
+    TD_HIP(hipMemcpyAsync(d.w, w, kNParams * 4, hipMemcpyHostToDevice, ctx->stream));
+    TD_HIP(hipMemsetAsync(d.m1, 0, kNParams * 4, ctx->stream));
+    TD_HIP(hipMemsetAsync(d.m2, 0, kNParams * 4, ctx->stream));
+    hipLaunchKernelGGL(MLPToBf16Kernel, dim3((kNParams + 255) / 256), dim3(256), 0,
+                       ctx->stream, d.w, d.wb);
+    TD_HIP(hipGetLastError());
+    ctx->params["mlp.adam_t"] = 0;
+    TD_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int td_mlp_get_weights(td_ctx* ctx, float* w, int64_t count) {
+  return Guard(ctx, [&] {
+    TD_REQUIRE(w && count == kNParams, TD_E_INVALID, "need td_mlp_num_params() floats");
+    DevNet d = NetBuffers(ctx);
+    TD_HIP(hipMemcpyAsync(w, d.w, kNParams * 4, hipMemcpyDeviceToHost, ctx->stream));
+    TD_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int td_mlp_features(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint64_t* adj64,
+                    float* feat_out) {
+  return Guard(ctx, [&] {
+    CheckGraphs(ngraphs, ns, adj64);
+    Uploaded u = UploadGraphs(ctx, "mlp.in", ngraphs, ns, adj64);
+    TD_HIP(hipMemcpyAsync(feat_out, u.feat, 1024ull * ngraphs, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TD_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int td_mlp_scores(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint64_t* adj64,
+                  float* scores_out) {
+  return Guard(ctx, [&] {
+    CheckGraphs(ngraphs, ns, adj64);
+    DevNet d = NetBuffers(ctx);
+    Uploaded u = UploadGraphs(ctx, "mlp.in", ngraphs, ns, adj64);
+    auto* scratch = static_cast<uint16_t*>(
+        ctx->Buffer("mlp.scratch", kScratchBf16 * 2 * static_cast<size_t>(ngraphs)));
+    auto* sc = static_cast<float*>(ctx->Buffer("mlp.scores", 256ull * ngraphs));
+    hipLaunchKernelGGL(MLPScoreKernel, dim3(ngraphs), dim3(64), 0, ctx->stream,
+                       Net{d.w, d.wb}, u.ns, u.adj, u.feat, scratch, sc);
+    TD_HIP(hipGetLastError());
+    TD_HIP(hipMemcpyAsync(scores_out, sc, 256ull * ngraphs, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TD_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+// Mean loss over the graphs and (grad_out non-null) its gradient.
+int td_mlp_loss_grad(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint64_t* adj64,
+                     const uint64_t* targets, float* loss_out, float* grad_out) {
+  return Guard(ctx, [&] {
+    CheckGraphs(ngraphs, ns, adj64);
+    TD_REQUIRE(targets != nullptr, TD_E_INVALID, "targets is null");
+    for (int i = 0; i < ngraphs; ++i) {
+      const uint64_t V = ns[i] >= 64 ? ~0ull : ((1ull << ns[i]) - 1ull);
+      TD_REQUIRE(targets[i] && !(targets[i] & ~V), TD_E_INVALID,
+                 "each target must be a nonempty subset of the graph's vertices");
+    }
+    DevNet d = NetBuffers(ctx);
+    Uploaded u = UploadGraphs(ctx, "mlp.in", ngraphs, ns, adj64);
+    auto* tg = static_cast<uint64_t*>(ctx->Buffer("mlp.in.tgt", 8ull * ngraphs));
+    TD_HIP(hipMemcpyAsync(tg, targets, 8ull * ngraphs, hipMemcpyHostToDevice, ctx->stream));
+    auto* scratch = static_cast<uint16_t*>(
+        ctx->Buffer("mlp.scratch", kScratchBf16 * 2 * static_cast<size_t>(ngraphs)));
+    auto* scf = static_cast<float*>(
+        ctx->Buffer("mlp.scratchf", kScratchF32 * 4 * static_cast<size_t>(ngraphs)));
+    auto* part = static_cast<float*>(
+        ctx->Buffer("mlp.part", 4ull * kNParams * static_cast<size_t>(ngraphs)));
+    auto* loss = static_cast<float*>(ctx->Buffer("mlp.loss", 4ull * ngraphs));
+    auto* grad = static_cast<float*>(ctx->Buffer("mlp.grad", 4ull * kNParams));
+    hipLaunchKernelGGL(MLPTrainKernel, dim3(ngraphs), dim3(64), 0, ctx->stream,
+                       Net{d.w, d.wb}, u.ns, u.adj, tg, u.feat, nullptr, scratch, scf, part,
+                       loss, 1.0f / static_cast<float>(ngraphs));
+    hipLaunchKernelGGL(MLPReduceKernel, dim3((kNParams + 255) / 256), dim3(256), 0,
+                       ctx->stream, part, ngraphs, grad);
+    TD_HIP(hipGetLastError());
+    std::vector<float> l(ngraphs);
+    TD_HIP(hipMemcpyAsync(l.data(), loss, 4ull * ngraphs, hipMemcpyDeviceToHost, ctx->stream));
+    if (grad_out)
+      TD_HIP(hipMemcpyAsync(grad_out, grad, 4ull * kNParams, hipMemcpyDeviceToHost,
+                            ctx->stream));
+    TD_HIP(hipStreamSynchronize(ctx->stream));
+    double s = 0;
+    for (float x : l) s += x;
+    if (loss_out) *loss_out = static_cast<float>(s / ngraphs);
+  });
+}
+
+// Adam over shuffled minibatches, all on the device; epoch_loss_out[e] =
+// mean training loss of epoch e.
+int td_mlp_train(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint64_t* adj64,
+                 const uint64_t* targets, int32_t epochs, int32_t batch, float lr,
+                 uint64_t seed, float* epoch_loss_out) {
+  return Guard(ctx, [&] {
+    CheckGraphs(ngraphs, ns, adj64);
+    TD_REQUIRE(targets && epochs >= 1 && batch >= 1 && lr > 0.f, TD_E_INVALID,
+               "bad training arguments");
+    for (int i = 0; i < ngraphs; ++i) {
+      const uint64_t V = ns[i] >= 64 ? ~0ull : ((1ull << ns[i]) - 1ull);
+      TD_REQUIRE(targets[i] && !(targets[i] & ~V), TD_E_INVALID,
+                 "each target must be a nonempty subset of the graph's vertices");
+    }
+    batch = std::min(batch, ngraphs);
+    DevNet d = NetBuffers(ctx);
+    Uploaded u = UploadGraphs(ctx, "mlp.train", ngraphs, ns, adj64);
+    auto* tg = static_cast<uint64_t*>(ctx->Buffer("mlp.train.tgt", 8ull * ngraphs));
+    TD_HIP(hipMemcpyAsync(tg, targets, 8ull * ngraphs, hipMemcpyHostToDevice, ctx->stream));
+    auto* scratch = static_cast<uint16_t*>(
+        ctx->Buffer("mlp.scratch", kScratchBf16 * 2 * static_cast<size_t>(batch)));
+    auto* scf = static_cast<float*>(
+        ctx->Buffer("mlp.scratchf", kScratchF32 * 4 * static_cast<size_t>(batch)));
+    auto* part = static_cast<float*>(
+        ctx->Buffer("mlp.part", 4ull * kNParams * static_cast<size_t>(batch)));
+    auto* loss = static_cast<float*>(
+        ctx->Buffer("mlp.trainloss", 4ull * static_cast<size_t>(ngraphs)));
+    auto* grad = static_cast<float*>(ctx->Buffer("mlp.grad", 4ull * kNParams));
+    auto* idx = static_cast<int32_t*>(ctx->Buffer("mlp.train.idx", 4ull * ngraphs));
+    std::mt19937_64 rng(seed);
+    std::vector<int32_t> perm(ngraphs);
+    for (int i = 0; i < ngraphs; ++i) perm[i] = i;
+    int64_t t = ctx->params["mlp.adam_t"];
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int e = 0; e < epochs; ++e) {
+      std::shuffle(perm.begin(), perm.end(), rng);
+      TD_HIP(hipMemcpyAsync(idx, perm.data(), 4ull * ngraphs, hipMemcpyHostToDevice,
+                            ctx->stream));
+      int nb = 0;
+      for (int s0 = 0; s0 + batch <= ngraphs; s0 += batch, ++nb) {
+        hipLaunchKernelGGL(MLPTrainKernel, dim3(batch), dim3(64), 0, ctx->stream,
+                           Net{d.w, d.wb}, u.ns, u.adj, tg, u.feat, idx + s0, scratch, scf,
+                           part, loss + s0, 1.0f / static_cast<float>(batch));
+        hipLaunchKernelGGL(MLPReduceKernel, dim3((kNParams + 255) / 256), dim3(256), 0,
+                           ctx->stream, part, batch, grad);
+        ++t;
+        const float c1 = 1.f - std::pow(b1, static_cast<float>(t));
+        const float c2 = 1.f - std::pow(b2, static_cast<float>(t));
+        hipLaunchKernelGGL(MLPAdamKernel, dim3((kNParams + 255) / 256), dim3(256), 0,
+                           ctx->stream, d.w, d.wb, d.m1, d.m2, grad, lr, b1, b2, eps, c1, c2);
+      }
+      TD_HIP(hipGetLastError());
+      std::vector<float> l(static_cast<size_t>(nb) * batch);
+      TD_HIP(hipMemcpyAsync(l.data(), loss, 4ull * l.size(), hipMemcpyDeviceToHost,
+                            ctx->stream));
+      TD_HIP(hipStreamSynchronize(ctx->stream));
+      double s = 0;
+      for (float x : l) s += x;
+      if (epoch_loss_out) epoch_loss_out[e] = l.empty() ? 0.f : static_cast<float>(s / l.size());
+    }
+    ctx->params["mlp.adam_t"] = t;
+  });
+}
+
+// Labelled samples from the last td_dp_solve: random connected vertex sets
+// (grown from a random vertex by random boundary vertices to a random size
+// in [2, n]); outputs the induced graph relabelled to 0..|S|-1 (64 rows per
+// sample), its optimal-root mask and td(S).
+int td_mlp_make_samples(td_ctx* ctx, int32_t count, uint64_t seed, int32_t* n_out,
+                        uint64_t* adj64_out, uint64_t* targets_out, uint8_t* td_out) {
+  return Guard(ctx, [&] {
+    TD_REQUIRE(ctx->have_dp, TD_E_STATE, "no td_dp_solve result to sample from");
+    TD_REQUIRE(count >= 1 && n_out && adj64_out && targets_out, TD_E_INVALID,
+               "bad arguments");
+    const DPGraph& G = ctx->last_graph;
+    const int n = static_cast<int>(G.n);
+    TD_REQUIRE(n >= 2, TD_E_STATE, "graph too small to sample");
+    std::mt19937_64 rng(seed);
+    std::vector<uint32_t> sets(count);
+    for (int i = 0; i < count; ++i) {
+      uint32_t S = 0;
+      for (int tries = 0; tries < 64 && __builtin_popcount(S) < 2; ++tries) {
+        const int target = 2 + static_cast<int>(rng() % static_cast<uint64_t>(n - 1));
+        const int start = static_cast<int>(rng() % static_cast<uint64_t>(n));
+        S = 1u << start;
+        while (__builtin_popcount(S) < target) {
+          uint32_t bd = 0;
+          for (uint32_t m = S; m; m &= m - 1) bd |= G.adj[__builtin_ctz(m)];
+          bd &= ~S;
+          if (!bd) break;
+          int pick = static_cast<int>(rng() % static_cast<uint64_t>(__builtin_popcount(bd)));
+          uint32_t b = bd;
+          while (pick--) b &= b - 1;
+          S |= b & (0u - b);
+        }
+      }
+      sets[i] = S;
+    }
+    auto* dsets = static_cast<uint32_t*>(ctx->Buffer("mlp.lab.sets", 4ull * count));
+    auto* dpos = static_cast<uint32_t*>(ctx->Buffer("mlp.lab.pos", 4ull * count));
+    auto* dtd = static_cast<uint8_t*>(ctx->Buffer("mlp.lab.td", static_cast<size_t>(count)));
+    TD_HIP(hipMemcpyAsync(dsets, sets.data(), 4ull * count, hipMemcpyHostToDevice, ctx->stream));
+    DPLookup L;
+    L.arena = static_cast<const uint8_t*>(ctx->bufs["dp.arena"].p);
+    L.lb = G.lb;
+    std::memcpy(L.lvl, G.lvl, sizeof(L.lvl));
+    const uint32_t* cb = static_cast<const uint32_t*>(ctx->bufs["dp.pascal"].p);
+    hipLaunchKernelGGL(LabelKernel, dim3(count), dim3(64), 0, ctx->stream, L, cb, dsets, dpos,
+                       dtd);
+    TD_HIP(hipGetLastError());
+    std::vector<uint32_t> pos(count);
+    std::vector<uint8_t> tds(count);
+    TD_HIP(hipMemcpyAsync(pos.data(), dpos, 4ull * count, hipMemcpyDeviceToHost, ctx->stream));
+    TD_HIP(hipMemcpyAsync(tds.data(), dtd, static_cast<size_t>(count), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TD_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < count; ++i) {
+      const uint32_t S = sets[i];
+      int map[32];
+      int k = 0;
+      for (int v = 0; v < 32; ++v) map[v] = (S >> v & 1u) ? k++ : -1;
+      n_out[i] = k;
+      uint64_t* rows = adj64_out + static_cast<size_t>(i) * 64;
+      for (int r = 0; r < 64; ++r) rows[r] = 0;
+      uint64_t tgt = 0;
+      for (int v = 0; v < n; ++v) {
+        if (map[v] < 0) continue;
+        uint64_t row = 0;
+        for (uint32_t m = G.adj[v] & S; m; m &= m - 1) row |= 1ull << map[__builtin_ctz(m)];
+        rows[map[v]] = row;
+        if (pos[i] >> v & 1u) tgt |= 1ull << map[v];
+      }
+      targets_out[i] = tgt;
+      if (td_out) td_out[i] = tds[i];
+    }
+  });
+}
+
+}  // extern "C"

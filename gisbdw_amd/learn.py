@@ -1,0 +1,142 @@
+"""The learned proposal: dataset from the exact DP, GPU training, scores.
+
+All arithmetic runs in libtdhip.so (csrc/mlp.hip): features, forward,
+backward and Adam are hand-written HIP kernels.  This module only moves
+arrays: it builds padded graph batches, asks the DP for optimal-root labels
+(td_mlp_make_samples), runs td_mlp_train and turns scores into the B&B's
+vertex_scores.  The scores order the branch-and-bound and seed its upper
+bound; they never prune.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+from pathlib import Path
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+from . import api
+from .graphs import Graph, SplitMix64, gnp
+
+P = C.POINTER
+
+
+def _p(a, t):
+    return a.ctypes.data_as(P(t))
+
+
+def pad_adj(graphs: Sequence[Graph]) -> Tuple[np.ndarray, np.ndarray]:
+    ns = np.array([g.n for g in graphs], np.int32)
+    adj = np.zeros((len(graphs), 64), np.uint64)
+    for i, g in enumerate(graphs):
+        adj[i, : g.n] = g.adj_array()
+    return ns, adj
+
+
+class Proposal:
+    """Weights live on the device in the context."""
+
+    def __init__(self, ctx: api.Context):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.nparams = self.lib.td_mlp_num_params()
+
+    def init(self, seed: int = 0):
+        self.ctx.check(self.lib.td_mlp_init(self.ctx.h, seed))
+
+    def get_weights(self) -> np.ndarray:
+        w = np.zeros(self.nparams, np.float32)
+        self.ctx.check(self.lib.td_mlp_get_weights(self.ctx.h, _p(w, C.c_float), w.size))
+        return w
+
+    def set_weights(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, np.float32)
+        self.ctx.check(self.lib.td_mlp_set_weights(self.ctx.h, _p(w, C.c_float), w.size))
+
+    def save(self, path):
+        np.save(path, self.get_weights(), allow_pickle=False)
+
+    def load(self, path):
+        self.set_weights(np.load(path, allow_pickle=False))
+
+    def features(self, ns, adj) -> np.ndarray:
+        out = np.zeros((len(ns), 64, 4), np.float32)
+        self.ctx.check(self.lib.td_mlp_features(self.ctx.h, len(ns), _p(ns, C.c_int32),
+                                                _p(adj, C.c_uint64), _p(out, C.c_float)))
+        return out
+
+    def scores_padded(self, ns, adj) -> np.ndarray:
+        out = np.zeros((len(ns), 64), np.float32)
+        self.ctx.check(self.lib.td_mlp_scores(self.ctx.h, len(ns), _p(ns, C.c_int32),
+                                              _p(adj, C.c_uint64), _p(out, C.c_float)))
+        return out
+
+    def scores(self, graphs: Sequence[Graph]) -> List[np.ndarray]:
+        ns, adj = pad_adj(graphs)
+        s = self.scores_padded(ns, adj)
+        return [s[i, : g.n].copy() for i, g in enumerate(graphs)]
+
+    def loss_grad(self, ns, adj, targets) -> Tuple[float, np.ndarray]:
+        loss = C.c_float()
+        grad = np.zeros(self.nparams, np.float32)
+        t = np.ascontiguousarray(targets, np.uint64)
+        self.ctx.check(self.lib.td_mlp_loss_grad(self.ctx.h, len(ns), _p(ns, C.c_int32),
+                                                 _p(adj, C.c_uint64), _p(t, C.c_uint64),
+                                                 C.byref(loss), _p(grad, C.c_float)))
+        return loss.value, grad
+
+    def train(self, ns, adj, targets, epochs=4, batch=256, lr=2e-3, seed=0) -> List[float]:
+        curve = np.zeros(epochs, np.float32)
+        t = np.ascontiguousarray(targets, np.uint64)
+        self.ctx.check(self.lib.td_mlp_train(self.ctx.h, len(ns), _p(ns, C.c_int32),
+                                             _p(adj, C.c_uint64), _p(t, C.c_uint64), epochs,
+                                             batch, lr, seed, _p(curve, C.c_float)))
+        return curve.tolist()
+
+
+def make_dataset(ctx: api.Context, n_graphs: int, samples_per_graph: int, seed: int = 0,
+                 nmin: int = 8, nmax: int = 24, pmin: float = 0.1, pmax: float = 0.5):
+    """Optimal-root labels from the exact DP on G(n, p), n ~ U{nmin..nmax},
+    p ~ U[pmin, pmax]: for random connected vertex sets S of each graph,
+    positives = {v : td(S \\ v) = td(S) - 1}."""
+    rng = SplitMix64(seed)
+    dp = api.DynamicGPU(ctx)
+    lib = ctx.lib
+    NS, ADJ, TGT, TD = [], [], [], []
+    for i in range(n_graphs):
+        n = nmin + rng.randbelow(nmax - nmin + 1)
+        p = pmin + (pmax - pmin) * rng.random()
+        g = gnp(n, p, rng.next_u64())
+        dp(g)
+        k = samples_per_graph
+        ns = np.zeros(k, np.int32)
+        adj = np.zeros((k, 64), np.uint64)
+        tgt = np.zeros(k, np.uint64)
+        td = np.zeros(k, np.uint8)
+        ctx.check(lib.td_mlp_make_samples(ctx.h, k, rng.next_u64(), _p(ns, C.c_int32),
+                                          _p(adj, C.c_uint64), _p(tgt, C.c_uint64),
+                                          _p(td, C.c_uint8)))
+        NS.append(ns)
+        ADJ.append(adj)
+        TGT.append(tgt)
+        TD.append(td)
+    return (np.concatenate(NS), np.concatenate(ADJ), np.concatenate(TGT),
+            np.concatenate(TD))
+
+
+def top1_accuracy(scores_padded: np.ndarray, ns: np.ndarray, targets: np.ndarray) -> float:
+    hit = 0
+    for i in range(len(ns)):
+        v = int(np.argmax(scores_padded[i, : ns[i]]))
+        hit += int(targets[i]) >> v & 1
+    return hit / len(ns)
+
+
+def degree_scores(ns: np.ndarray, adj: np.ndarray) -> np.ndarray:
+    """The plain ordering: degree, ties to the lower vertex id."""
+    out = np.full((len(ns), 64), -1e9, np.float32)
+    for i in range(len(ns)):
+        for v in range(ns[i]):
+            out[i, v] = bin(int(adj[i, v])).count("1") - v * 1e-3
+    return out

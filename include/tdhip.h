@@ -143,6 +143,44 @@ int td_bnb_solve_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
                        int8_t* parent_concat, uint8_t* certified_out,
                        td_bnb_stats* stats);
 
+/* ---------------------------------------------------------------------- */
+/* Learned proposal (no reference counterpart: it feeds the reference's     */
+/* LowerBoundInfo::attempt_order and Heuristic::Get extension points,       */
+/* Treedepth/src/branch_and_bound/branch_and_bound.hpp:21-24, :38-42)       */
+/* ---------------------------------------------------------------------- */
+/* A message-passing MLP over per-vertex features (degree, core number,
+ * vertices within distance 2, BFS eccentricity): 4 -> 64, two rounds of
+ * mean aggregation (64+64 -> 64), then 64 -> 64 -> 64 -> 1.  BF16 weights and
+ * activations on MFMA, FP32 accumulation, FP32 master weights + Adam.
+ * Graph batches use 64 adjacency rows per graph (rows >= n zero). */
+int td_mlp_num_params(void);
+int td_mlp_init(td_ctx* ctx, uint64_t seed);
+int td_mlp_set_weights(td_ctx* ctx, const float* w, int64_t count);
+int td_mlp_get_weights(td_ctx* ctx, float* w, int64_t count);
+/* features_out: ngraphs x 64 x 4 floats (rows >= n zero) */
+int td_mlp_features(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
+                    const uint64_t* adj64, float* features_out);
+/* scores_out: ngraphs x 64 floats (higher = better root) */
+int td_mlp_scores(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
+                  const uint64_t* adj64, float* scores_out);
+/* targets[i]: mask of optimal roots of graph i.  loss_out = mean
+ * cross-entropy; grad_out (nullable): its gradient, td_mlp_num_params()
+ * floats, ordered like the weights. */
+int td_mlp_loss_grad(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
+                     const uint64_t* adj64, const uint64_t* targets,
+                     float* loss_out, float* grad_out);
+/* Adam on shuffled minibatches, entirely on the device. */
+int td_mlp_train(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
+                 const uint64_t* adj64, const uint64_t* targets, int32_t epochs,
+                 int32_t batch, float lr, uint64_t seed,
+                 float* epoch_loss_out);
+/* Labelled samples from the last td_dp_solve: random connected vertex sets S
+ * (induced graph relabelled, 64 rows each), their optimal-root masks
+ * {v : td(S \ v) = td(S) - 1} and td(S). */
+int td_mlp_make_samples(td_ctx* ctx, int32_t count, uint64_t seed,
+                        int32_t* n_out, uint64_t* adj64_out,
+                        uint64_t* targets_out, uint8_t* td_out);
+
 #ifdef __cplusplus
 }
 #endif

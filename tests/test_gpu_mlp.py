@@ -1,0 +1,123 @@
+"""GPU learned proposal (td_mlp_* in include/tdhip.h) against the CPU FP64
+reference (tests/mlp_ref.py, itself pinned by finite differences).
+
+Tolerances: features are integers scaled by powers of two, so bit-exact.
+Scores and gradients go through bf16 MFMA inputs; the reference rounds at
+the same points, so the remaining difference is FP32-vs-FP64 accumulation
+and rare bf16 rounding-boundary flips: relative L2 error <= 2e-2 per
+parameter block, cosine >= 0.999."""
+import math
+import random
+
+import numpy as np
+import pytest
+
+import mlp_ref
+from gisbdw_amd import api, learn
+from gisbdw_amd import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_graphs(seed, count, nmin=3, nmax=30):
+    rng = random.Random(seed)
+    gs = [G.gnp(rng.randint(nmin, nmax), rng.uniform(0.1, 0.5), rng.getrandbits(32))
+          for _ in range(count)]
+    gs += [G.path(9), G.cycle(12), G.star(7), G.random_cubic(40, 3), G.tree_plus_edges(64, 0.1, 2)]
+    return gs
+
+
+def test_features_bit_exact(ctx):
+    prop = learn.Proposal(ctx)
+    gs = rand_graphs(1, 12)
+    ns, adj = learn.pad_adj(gs)
+    got = prop.features(ns, adj)
+    for i, g in enumerate(gs):
+        want = mlp_ref.features(g.n, adj[i])
+        assert np.array_equal(got[i].astype(np.float64), want), g.name
+
+
+def test_scores_and_grad_match_reference(ctx):
+    prop = learn.Proposal(ctx)
+    w = mlp_ref.init_weights(7).astype(np.float32)
+    prop.set_weights(w)
+    assert np.array_equal(prop.get_weights(), w)
+    gs = rand_graphs(2, 6, 4, 24)
+    ns, adj = learn.pad_adj(gs)
+    rng = random.Random(3)
+    tg = []
+    for g in gs:
+        t = 0
+        while t == 0:
+            t = sum(1 << v for v in range(g.n) if rng.random() < 0.3)
+        tg.append(t)
+    tg = np.array(tg, np.uint64)
+    scores = prop.scores_padded(ns, adj)
+    wref = w.astype(np.float64)
+    for i, g in enumerate(gs):
+        _, _, s = mlp_ref.forward_backward(wref, mlp_ref.features(g.n, adj[i]), g.n, adj[i],
+                                           int(tg[i]), emulate=True)
+        err = np.abs(scores[i, : g.n] - s).max() / (np.abs(s).max() + 1e-6)
+        assert err < 2e-2, (g.name, err)
+    loss, grad = prop.loss_grad(ns, adj, tg)
+    rl, rg = mlp_ref.batch_loss_grad(wref, [(g.n, adj[i], int(tg[i])) for i, g in enumerate(gs)])
+    assert abs(loss - rl) <= 1e-2 * abs(rl) + 1e-3, (loss, rl)
+    for k, sh in mlp_ref.SHAPES.items():
+        a = grad[mlp_ref.OFF[k]: mlp_ref.OFF[k] + int(np.prod(sh))].astype(np.float64)
+        b = rg[mlp_ref.OFF[k]: mlp_ref.OFF[k] + int(np.prod(sh))]
+        nb = np.linalg.norm(b)
+        if nb < 1e-7:
+            assert np.linalg.norm(a) < 1e-5, k
+            continue
+        rel = np.linalg.norm(a - b) / nb
+        cos = float(a @ b / (np.linalg.norm(a) * nb + 1e-30))
+        print(k, rel, cos, flush=True)
+        assert rel < 2e-2 and cos > 0.999, (k, rel, cos)
+
+
+def test_labels_are_optimal_roots(ctx, oracle):
+    dp = api.DynamicGPU(ctx)
+    g = G.gnp(16, 0.3, 4)
+    dp(g)
+    import ctypes as C
+
+    k = 40
+    ns = np.zeros(k, np.int32)
+    adj = np.zeros((k, 64), np.uint64)
+    tg = np.zeros(k, np.uint64)
+    td = np.zeros(k, np.uint8)
+    P = C.POINTER
+    ctx.check(ctx.lib.td_mlp_make_samples(ctx.h, k, 5, ns.ctypes.data_as(P(C.c_int32)),
+                                          adj.ctypes.data_as(P(C.c_uint64)),
+                                          tg.ctypes.data_as(P(C.c_uint64)),
+                                          td.ctypes.data_as(P(C.c_uint8))))
+    for i in range(k):
+        sub = G.Graph(int(ns[i]), [int(a) for a in adj[i, : ns[i]]])
+        sub.validate()
+        assert sub.is_connected()
+        t = oracle.dp_fast(sub)[0]
+        assert t == td[i]
+        pos = 0
+        for v in range(sub.n):
+            keep = [u for u in range(sub.n) if u != v]
+            rest = G.Graph(sub.n - 1, [sum(1 << keep.index(u) for u in keep if sub.adj[w] >> u & 1)
+                                       for w in keep]) if sub.n > 1 else None
+            tv = oracle.dp_fast(rest)[0] if rest is not None and rest.n > 0 else 0
+            if tv == t - 1:
+                pos |= 1 << v
+        assert pos == int(tg[i]), i
+
+
+def test_training_learns_and_beats_degree_ordering(ctx):
+    prop = learn.Proposal(ctx)
+    prop.init(1)
+    ns, adj, tg, td = learn.make_dataset(ctx, 160, 100, seed=11, nmin=8, nmax=18)
+    order = np.random.default_rng(0).permutation(len(ns))
+    tr, te = order[:14000], order[14000:]
+    curve = prop.train(ns[tr], adj[tr], tg[tr], epochs=5, batch=256, lr=2e-3, seed=3)
+    print("loss curve", curve, flush=True)
+    assert curve[-1] < curve[0]
+    acc = learn.top1_accuracy(prop.scores_padded(ns[te], adj[te]), ns[te], tg[te])
+    base = learn.top1_accuracy(learn.degree_scores(ns[te], adj[te]), ns[te], tg[te])
+    print("top-1 optimal root: learned", acc, "degree", base, flush=True)
+    assert acc > base
