@@ -45,7 +45,58 @@ def parse():
     ap.add_argument("--low-bits", type=int, default=None)
     ap.add_argument("--tiles-per-lane", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
+    ap.add_argument("--no-bnb", action="store_true",
+                    help="skip the branch-and-bound / learned-proposal extra figures")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel over MIB MiB (rocprof calibration)")
     return ap.parse_args()
+
+
+def bnb_extra(ctx, budget=200000):
+    """Branch-and-bound figures for the extra block: the 256-graph sparse
+    workload with the plain search, then 1024 graphs of the same distribution
+    plain vs learned (proposal trained here on the GPU on DP labels).
+    certified graphs/s = graphs proven optimal / wall time of the call."""
+    from gisbdw_amd import api, learn
+    from gisbdw_amd import graphs as G
+
+    ctx.set_param("bnb.node_budget", budget)
+    out = {"node_budget": budget}
+    W = G.workload_bnb(256, 0)
+    t0 = time.perf_counter()
+    tds, pars, cert, st = api.bnb_solve_batch(ctx, W)
+    wall = time.perf_counter() - t0
+    out["bnb256_plain"] = {"certified": int(sum(cert)), "graphs": len(W), "wall_s": wall,
+                           "certified_graphs_per_s": sum(cert) / wall, "nodes": st.nodes,
+                           "kernel_ms": st.kernel_ms}
+    prop = learn.Proposal(ctx)
+    prop.init(1)
+    t0 = time.perf_counter()
+    ns, adj, tg, _ = learn.make_dataset(ctx, 800, 250, seed=11, nmin=8, nmax=24)
+    t1 = time.perf_counter()
+    curve = prop.train(ns, adj, tg, epochs=6, batch=512, lr=2e-3, seed=3)
+    t2 = time.perf_counter()
+    out["proposal_training"] = {"labels": int(len(ns)), "label_s": t1 - t0, "train_s": t2 - t1,
+                                "epoch_loss": [round(x, 4) for x in curve]}
+    W = G.workload_bnb(1024, 1000)
+    t0 = time.perf_counter()
+    tds, pars, cert, st = api.bnb_solve_batch(ctx, W)
+    wall_p = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    scores = prop.scores(W)
+    tdl, parl, certl, stl = api.bnb_solve_batch(ctx, W, scores=scores)
+    wall_l = time.perf_counter() - t0
+    agree = all(a == b for a, b, c, d in zip(tds, tdl, cert, certl) if c and d)
+    out["bnb1024"] = {
+        "plain": {"certified": int(sum(cert)), "wall_s": wall_p,
+                  "certified_graphs_per_s": sum(cert) / wall_p, "nodes": st.nodes},
+        "learned": {"certified": int(sum(certl)), "wall_s": wall_l,
+                    "certified_graphs_per_s": sum(certl) / wall_l, "nodes": stl.nodes,
+                    "includes_scoring": True},
+        "learned_over_plain": (sum(certl) / wall_l) / max(sum(cert) / wall_p, 1e-9),
+        "certified_answers_agree": agree,
+    }
+    return out
 
 
 def cpu_baseline_dp(g, target_s=12.0):
@@ -79,14 +130,14 @@ def cpu_baseline_dp(g, target_s=12.0):
 
 
 def profile_traffic():
-    """HBM bytes per level launch from the committed rocprofv3 --pmc summary,
-    if present (see profiles/)."""
-    p = ROOT / "profiles" / "dp_level_kernel_n30_traffic.json"
+    """HBM bytes per level launch from the committed rocprofv3 --pmc summary
+    (profiles/dp_level_kernel_n30_traffic.md, calibrated against a plain copy
+    of known size), or None when the summary is absent."""
+    p = ROOT / "profiles" / "dp_level_kernel_n30_traffic.md"
     if p.exists():
-        try:
-            return json.loads(p.read_text()).get("traffic_bytes_per_launch")
-        except Exception:
-            return None
+        for line in p.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
     return None
 
 
@@ -100,6 +151,10 @@ def main():
     from gisbdw_amd import graphs as G
 
     ctx = api.Context(0)
+    if args.calibrate_copy:
+        gbps = ctx.copy_bench(args.calibrate_copy << 20, 3)
+        print(json.dumps({"copy_MiB": args.calibrate_copy, "launches": 4, "GBps": gbps}))
+        return
     if args.low_bits:
         ctx.set_param("dp.low_bits", args.low_bits)
     if args.tiles_per_lane:
@@ -214,6 +269,10 @@ def main():
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
             "extra": {"subsets_per_s": sub / wall},
         }
+    out["extra"]["copy_GBps_2GiB"] = ctx.copy_bench(2 << 30, 3)
+    if not args.no_bnb and args.workload == "dp30":
+        print("[bench] branch-and-bound extra", file=sys.stderr, flush=True)
+        out["extra"]["bnb"] = bnb_extra(ctx)
     if not args.no_cpu:
         g0 = graphs[0]
         out["cpu_baseline"] = cpu_baseline_dp(g0)

@@ -97,6 +97,7 @@ def load_library():
         "td_dp_solve_batch": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
                                         P(C.c_uint8), P(C.c_int8), C.c_int32, P(DPStats)]),
         "td_dp_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, P(C.c_double)]),
+        "td_ctx_copy_bench": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, P(C.c_double)]),
     }
     sig.update({
         "td_bnb_solve_batch": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
@@ -176,6 +177,12 @@ class Context:
 
     def set_param(self, key: str, value: int):
         self.check(self.lib.td_ctx_set_param(self.h, key.encode(), int(value)))
+
+    def copy_bench(self, nbytes: int, reps: int = 5) -> float:
+        """Achieved GB/s of a plain device copy (read + write)."""
+        v = C.c_double()
+        self.check(self.lib.td_ctx_copy_bench(self.h, nbytes, reps, C.byref(v)))
+        return v.value
 
     def get_param(self, key: str) -> int:
         v = C.c_int64()

@@ -105,7 +105,48 @@ void td_ctx::Release(const std::string& name) {
   bufs.erase(it);
 }
 
+namespace {
+// A plain streaming copy, 16 bytes per lane per step: the reference point for
+// HBM bandwidth and for scaling rocprof's FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void CopyKernel(const uint4* __restrict__ src,
+                                                  uint4* __restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
 extern "C" {
+
+int td_ctx_copy_bench(td_ctx* c, uint64_t bytes, int32_t reps, double* gbps_out) {
+  if (!c || bytes < 16 || reps < 1) return TD_E_INVALID;
+  try {
+    TD_HIP(hipSetDevice(c->device));
+    bytes &= ~15ull;
+    void* a = c->Buffer("copy.src", bytes);
+    void* b = c->Buffer("copy.dst", bytes);
+    const size_t n = bytes / 16;
+    const int grid = c->prop.multiProcessorCount * 8;
+    hipLaunchKernelGGL(CopyKernel, dim3(grid), dim3(256), 0, c->stream,
+                       static_cast<const uint4*>(a), static_cast<uint4*>(b), n);
+    TD_HIP(hipEventRecord(c->ev[0], c->stream));
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(CopyKernel, dim3(grid), dim3(256), 0, c->stream,
+                         static_cast<const uint4*>(a), static_cast<uint4*>(b), n);
+    TD_HIP(hipEventRecord(c->ev[1], c->stream));
+    TD_HIP(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    TD_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    if (gbps_out) *gbps_out = 2.0 * bytes * reps / (ms * 1e-3) / 1e9;
+    c->Release("copy.src");
+    c->Release("copy.dst");
+    c->err.clear();
+    return TD_OK;
+  } catch (const tdhip::Error& e) {
+    c->err = e.msg;
+    return e.code;
+  }
+}
 
 int td_ctx_create(td_ctx** out, int device, uint64_t mem_budget_bytes) {
   if (!out) return TD_E_INVALID;

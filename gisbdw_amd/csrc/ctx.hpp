@@ -48,8 +48,8 @@ struct DPGraph {
 
 // DP launch configuration (autotuned).
 struct DPConfig {
-  int low_bits = 4;        // 2..5
-  int tiles_per_lane = 4;  // consecutive tiles per lane (Gosper steps)
+  int low_bits = 5;        // 2..5 (autotuned on MI355X at n = 30)
+  int tiles_per_lane = 1;  // consecutive tiles per lane (Gosper steps)
   int block = 256;         // threads per workgroup
   int batch_wg_per_cu = 4; // persistent batch kernel occupancy target
   int use_graph = 1;       // capture the level sequence into a hipGraph

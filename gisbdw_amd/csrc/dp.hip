@@ -33,6 +33,7 @@ namespace {
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kCbStride = 33;  // s_cb[x * 33 + j] = C(x, j), x < 32, j <= 32
+constexpr int kNbUnroll = 4;   // neighbour tiles loaded per step
 
 __device__ __forceinline__ u16x2 AsU16x2(uint32_t x) {
   return __builtin_bit_cast(u16x2, x);
@@ -122,27 +123,40 @@ __device__ __forceinline__ void DPTile(const uint8_t* __restrict__ prev,
   }
   // neighbour tiles H \ x_j, j = h-1 .. 0 (x_j = j-th smallest of H):
   // rank(H \ x_j) = t - C(x_j, j+1) - sum_{i>j} (C(x_i, i+1) - C(x_i, i)).
+  // Four neighbour tiles per step: h is uniform across the wave, so the
+  // guards below are uniform and the four loads are in flight together.
   uint32_t rem = H;
   uint32_t A = 0;
-  for (int j = h - 1; j >= 0; --j) {
-    const int x = 31 - __builtin_clz(rem);
-    rem ^= 1u << x;
-    const uint32_t c1 = s_cb[x * kCbStride + j + 1];
-    const uint32_t c0 = s_cb[x * kCbStride + j];
-    const uint32_t r2 = t - c1 - A;
-    A += c1 - c0;
-    uint32_t wv[NW];
-    LoadTile<LB>(prev + (static_cast<size_t>(r2) << LB), wv);
+  for (int j0 = h - 1; j0 >= 0; j0 -= kNbUnroll) {
+    uint32_t wv[kNbUnroll][NW];
 #pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      const uint32_t x = wv[w];
-      const u16x2 xe = AsU16x2(__builtin_amdgcn_perm(x, x, 0x0c020c00u));
-      const u16x2 xo = AsU16x2(__builtin_amdgcn_perm(x, x, 0x0c030c01u));
-      mnE[w] = __builtin_elementwise_min(mnE[w], xe);
-      mnO[w] = __builtin_elementwise_min(mnO[w], xo);
-      mxE[w] = __builtin_elementwise_max(mxE[w], xe);
-      mxO[w] = __builtin_elementwise_max(mxO[w], xo);
-      orr[w] |= x;
+    for (int u = 0; u < kNbUnroll; ++u) {
+      const int j = j0 - u;
+      if (j >= 0) {
+        const int x = 31 - __builtin_clz(rem);
+        rem ^= 1u << x;
+        const uint32_t c1 = s_cb[x * kCbStride + j + 1];
+        const uint32_t c0 = s_cb[x * kCbStride + j];
+        const uint32_t r2 = t - c1 - A;
+        A += c1 - c0;
+        LoadTile<LB>(prev + (static_cast<size_t>(r2) << LB), wv[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kNbUnroll; ++u) {
+      if (j0 - u >= 0) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+          const uint32_t x = wv[u][w];
+          const u16x2 xe = AsU16x2(__builtin_amdgcn_perm(x, x, 0x0c020c00u));
+          const u16x2 xo = AsU16x2(__builtin_amdgcn_perm(x, x, 0x0c030c01u));
+          mnE[w] = __builtin_elementwise_min(mnE[w], xe);
+          mnO[w] = __builtin_elementwise_min(mnO[w], xo);
+          mxE[w] = __builtin_elementwise_max(mxE[w], xe);
+          mxO[w] = __builtin_elementwise_max(mxO[w], xo);
+          orr[w] |= x;
+        }
+      }
     }
   }
   // N(H) from the 7-bit chunk tables

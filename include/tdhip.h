@@ -40,11 +40,18 @@ int td_ctx_create(td_ctx** out, int device, uint64_t mem_budget_bytes);
 void td_ctx_destroy(td_ctx* ctx);
 /* Last error message of this context ("" if none). */
 const char* td_last_error(const td_ctx* ctx);
-/* Tuning knobs (see td_dp_autotune): "dp.low_bits" (3..5), "dp.tiles_per_lane"
- * (1..16), "dp.block" (64..1024), "dp.batch_wg_per_cu", "dp.use_graph" (0/1),
- * "bnb.waves", "bnb.node_budget", "bnb.time_budget_ms", "bnb.memo_log2". */
+/* Tuning knobs (see td_dp_autotune): "dp.low_bits" (2..5, default 5),
+ * "dp.tiles_per_lane" (1..64, default 1), "dp.block" (64..1024, default 256),
+ * "dp.batch_wg_per_cu", "dp.use_graph" (0/1), "bnb.waves", "bnb.node_budget"
+ * (default 2e6), "bnb.memo_log2" (default 19). */
 int td_ctx_set_param(td_ctx* ctx, const char* key, int64_t value);
 int td_ctx_get_param(const td_ctx* ctx, const char* key, int64_t* value);
+
+/* Streaming copy of `bytes` (read + write), reps times: returns the achieved
+ * GB/s (2 * bytes * reps / time).  No reference counterpart: it measures the
+ * device's practical HBM bandwidth and is the known-size kernel used to scale
+ * rocprofv3's FETCH_SIZE / WRITE_SIZE counters. */
+int td_ctx_copy_bench(td_ctx* ctx, uint64_t bytes, int32_t reps, double* gbps_out);
 
 /* ---------------------------------------------------------------------- */
 /* Subset DP                                                               */

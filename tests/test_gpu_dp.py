@@ -97,9 +97,10 @@ def test_golden_cases(dp, oracle):
         assert oracle.certify(g, dp.parent, dp.treedepth)
 
 
-@pytest.mark.parametrize("low_bits", [2, 3, 4, 5])
-def test_all_levels_bit_exact(ctx, oracle, low_bits):
+@pytest.mark.parametrize("low_bits,tiles", [(2, 4), (3, 1), (4, 4), (5, 1), (5, 3)])
+def test_all_levels_bit_exact(ctx, oracle, low_bits, tiles):
     ctx.set_param("dp.low_bits", low_bits)
+    ctx.set_param("dp.tiles_per_lane", tiles)
     try:
         dp = api.DynamicGPU(ctx)
         rng = random.Random(1234 + low_bits)
@@ -116,7 +117,8 @@ def test_all_levels_bit_exact(ctx, oracle, low_bits):
                 got = dp.level(k)
                 assert np.array_equal(got, lv[off[k]: off[k + 1]]), (n, k)
     finally:
-        ctx.set_param("dp.low_bits", 4)
+        ctx.set_param("dp.low_bits", 5)
+        ctx.set_param("dp.tiles_per_lane", 1)
 
 
 def test_queries_match_reference_semantics(dp, oracle):
