@@ -266,18 +266,30 @@ class DynamicGPU:
         return out[: comb(self.n, k)]
 
 
+def pack_graphs(graphs: Sequence[Graph]):
+    """(n array, concatenated adjacency rows) as the batch entry points take
+    them."""
+    ns = np.array([g.n for g in graphs], np.int32)
+    adj = np.concatenate([g.adj_array() for g in graphs]).astype(np.uint64)
+    return ns, adj
+
+
+def dp_solve_batch_arrays(ctx: Context, ns: np.ndarray, adj: np.ndarray, mode: int = 0):
+    """Batch DP on packed arrays; returns (td array, parent array, stats)."""
+    td = np.zeros(len(ns), np.uint8)
+    par = np.zeros(int(ns.sum()), np.int8)
+    st = DPStats()
+    ctx.check(ctx.lib.td_dp_solve_batch(ctx.h, len(ns), _ptr(ns, C.c_int32),
+                                        _ptr(adj, C.c_uint64), _ptr(td, C.c_uint8),
+                                        _ptr(par, C.c_int8), mode, C.byref(st)))
+    return td, par, st
+
+
 def dp_solve_batch(ctx: Context, graphs: Sequence[Graph], mode: int = 0):
     """td::DynamicGPU over many graphs in one call.  Returns (treedepths,
     parents, stats)."""
-    lib = ctx.lib
-    ns = np.array([g.n for g in graphs], np.int32)
-    adj = np.concatenate([g.adj_array() for g in graphs]).astype(np.uint64)
-    td = np.zeros(len(graphs), np.uint8)
-    par = np.zeros(int(ns.sum()), np.int8)
-    st = DPStats()
-    ctx.check(lib.td_dp_solve_batch(ctx.h, len(graphs), _ptr(ns, C.c_int32),
-                                    _ptr(adj, C.c_uint64), _ptr(td, C.c_uint8),
-                                    _ptr(par, C.c_int8), mode, C.byref(st)))
+    ns, adj = pack_graphs(graphs)
+    td, par, st = dp_solve_batch_arrays(ctx, ns, adj, mode)
     parents = []
     off = 0
     for g in graphs:

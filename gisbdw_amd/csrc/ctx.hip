@@ -211,8 +211,13 @@ int td_ctx_set_param(td_ctx* c, const char* key, int64_t v) {
   } else if (k == "dp.batch_wg_per_cu") {
     if (v < 1 || v > 32) return bad("must be in [1, 32]");
     c->dp.batch_wg_per_cu = (int)v;
+  } else if (k == "dp.batch_block") {
+    if (v < 64 || v > 1024 || v % 64) return bad("must be a multiple of 64 in [64, 1024]");
+    c->dp.batch_block = (int)v;
   } else if (k == "dp.use_graph") {
     c->dp.use_graph = v ? 1 : 0;
+  } else if (k == "dp.xcd_remap") {
+    c->dp.xcd_remap = v ? 1 : 0;
   } else if (k.rfind("bnb.", 0) == 0 || k.rfind("mlp.", 0) == 0) {
     c->params[k] = v;
   } else {
@@ -231,7 +236,9 @@ int td_ctx_get_param(const td_ctx* c, const char* key, int64_t* v) {
   else if (k == "dp.tiles_per_lane") *v = c->dp.tiles_per_lane;
   else if (k == "dp.block") *v = c->dp.block;
   else if (k == "dp.batch_wg_per_cu") *v = c->dp.batch_wg_per_cu;
+  else if (k == "dp.batch_block") *v = c->dp.batch_block;
   else if (k == "dp.use_graph") *v = c->dp.use_graph;
+  else if (k == "dp.xcd_remap") *v = c->dp.xcd_remap;
   else {
     auto it = c->params.find(k);
     if (it == c->params.end()) return TD_E_INVALID;

@@ -51,8 +51,10 @@ struct DPConfig {
   int low_bits = 5;        // 2..5 (autotuned on MI355X at n = 30)
   int tiles_per_lane = 1;  // consecutive tiles per lane (Gosper steps)
   int block = 256;         // threads per workgroup
-  int batch_wg_per_cu = 4; // persistent batch kernel occupancy target
+  int batch_wg_per_cu = 4; // persistent batch kernel: workgroups per CU
+  int batch_block = 1024;  // persistent batch kernel: threads per workgroup
   int use_graph = 1;       // capture the level sequence into a hipGraph
+  int xcd_remap = 0;       // contiguous tile slice per XCD (measured slower)
 };
 
 struct DeviceBuffer {

@@ -238,15 +238,24 @@ __device__ __forceinline__ void DPLevelRange(const DPGraph* __restrict__ g,
 }
 
 // One high level of one graph.
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup i -> XCD i % 8).
+// With xcd_remap the tile range is cut into 8 contiguous slices and XCD x
+// works through slice x, so the neighbour tiles that consecutive tiles share
+// stay in one XCD's L2.  Needs gridDim.x % 8 == 0 (the host ensures it).
+constexpr uint32_t kXcds = 8;
+
 template <int LB>
 __global__ __launch_bounds__(1024) void DPLevelKernel(
     const DPGraph* __restrict__ g, const uint32_t* __restrict__ g_cb,
-    uint8_t* __restrict__ arena, int h, uint32_t ntiles, int R) {
+    uint8_t* __restrict__ arena, int h, uint32_t ntiles, int R, int xcd_remap) {
   __shared__ uint32_t s_cb[32 * kCbStride];
   __shared__ uint32_t s_nbh[512];
   LoadShared<LB>(g, g_cb, s_cb, s_nbh, true);
   __syncthreads();
-  const uint32_t lane_id = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t bid = blockIdx.x;
+  if (xcd_remap && gridDim.x % kXcds == 0)
+    bid = (bid % kXcds) * (gridDim.x / kXcds) + bid / kXcds;
+  const uint32_t lane_id = bid * blockDim.x + threadIdx.x;
   const uint32_t stride = gridDim.x * blockDim.x * R;
   DPLevelRange<LB>(g, arena, h, 0, ntiles, lane_id, stride, R, s_cb, s_nbh);
 }
@@ -277,7 +286,7 @@ __global__ __launch_bounds__(1024) void DPChunkKernel(
 // workgroup barrier between levels, so no level of a graph ever waits on
 // another workgroup and the graph's tables stay in one XCD's L2.
 template <int LB>
-__global__ __launch_bounds__(256) void DPPersistentKernel(
+__global__ __launch_bounds__(1024) void DPPersistentKernel(
     const DPGraph* __restrict__ gs, const uint32_t* __restrict__ g_cb,
     uint8_t* __restrict__ arena, int ngraphs, uint32_t* __restrict__ queue,
     int R) {
@@ -431,15 +440,16 @@ DPGraph MakeGraph(int n, const uint64_t* adj, int lb, uint64_t table_off) {
       if (L >> l & 1) m |= G.adj[l];
     G.nl[L] = m;
   }
-  for (int c = 0; c < 4; ++c)
-    for (int chunk = 0; chunk < 128; ++chunk) {
-      uint32_t m = 0;
-      for (int b = 0; b < 7; ++b) {
-        const int x = 7 * c + b;
-        if ((chunk >> b & 1) && x < static_cast<int>(G.nh)) m |= G.adj[lb + x];
-      }
-      G.nbh[128 * c + chunk] = m;
+  // N(chunk) = N(chunk without its lowest bit) | N(lowest bit)
+  for (int c = 0; c < 4; ++c) {
+    G.nbh[128 * c] = 0;
+    for (int chunk = 1; chunk < 128; ++chunk) {
+      const int b = __builtin_ctz(static_cast<unsigned>(chunk));
+      const int x = 7 * c + b;
+      const uint32_t row = x < static_cast<int>(G.nh) ? G.adj[lb + x] : 0u;
+      G.nbh[128 * c + chunk] = G.nbh[128 * c + (chunk & (chunk - 1))] | row;
     }
+  }
   return G;
 }
 
@@ -473,9 +483,10 @@ void LaunchLevels(td_ctx* ctx, const DPGraph& G, const DPGraph* dG,
   const int R = ctx->dp.tiles_per_lane, block = ctx->dp.block;
   for (int h = 0; h <= static_cast<int>(G.nh); ++h) {
     const uint32_t ntiles = static_cast<uint32_t>(Binom(G.nh, h));
-    const int grid = GridFor(ctx, (ntiles + R - 1) / R, block);
+    int grid = GridFor(ctx, (ntiles + R - 1) / R, block);
+    if (ctx->dp.xcd_remap && grid >= 64) grid = (grid + 7) / 8 * 8;
     hipLaunchKernelGGL(DPLevelKernel<LB>, dim3(grid), dim3(block), 0,
-                       ctx->stream, dG, cb, arena, h, ntiles, R);
+                       ctx->stream, dG, cb, arena, h, ntiles, R, ctx->dp.xcd_remap);
     ++*launches;
   }
   TD_HIP(hipGetLastError());
@@ -658,11 +669,42 @@ void LaunchBatch(td_ctx* ctx, int mode, int ngraphs,
                  uint64_t* launches) {
   const int R = ctx->dp.tiles_per_lane, block = ctx->dp.block;
   if (mode == 0) {
-    hipLaunchKernelGGL(ResetCounter, dim3(1), dim3(64), 0, ctx->stream, dqueue);
+    // queue reset (a kernel, not a memset, so the pair can be captured) and
+    // the persistent launch, replayed from a hipGraph cached per batch shape
     const int grid = ctx->prop.multiProcessorCount * ctx->dp.batch_wg_per_cu;
-    hipLaunchKernelGGL(DPPersistentKernel<LB>, dim3(grid), dim3(256), 0,
-                       ctx->stream, dGs, cb, arena, ngraphs, dqueue, R);
-    *launches += 2;
+    const int bblock = ctx->dp.batch_block;
+    auto launch = [&] {
+      hipLaunchKernelGGL(ResetCounter, dim3(1), dim3(64), 0, ctx->stream, dqueue);
+      hipLaunchKernelGGL(DPPersistentKernel<LB>, dim3(grid), dim3(bblock), 0,
+                         ctx->stream, dGs, cb, arena, ngraphs, dqueue, R);
+    };
+    if (!ctx->dp.use_graph) {
+      launch();
+      *launches += 2;
+    } else {
+      const td_ctx::GraphKey key{-ngraphs, LB, R, bblock * 64 + ctx->dp.batch_wg_per_cu};
+      const void* sig[3] = {arena, dGs, dqueue};
+      auto it = ctx->dp_graphs.find(key);
+      auto& p = ctx->dp_graph_ptrs[key];
+      if (it != ctx->dp_graphs.end() && (p.first != sig[0] || p.second != sig[1])) {
+        (void)hipGraphExecDestroy(it->second);
+        ctx->dp_graphs.erase(it);
+        it = ctx->dp_graphs.end();
+      }
+      if (it == ctx->dp_graphs.end()) {
+        hipGraph_t graph;
+        TD_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        launch();
+        TD_HIP(hipStreamEndCapture(ctx->stream, &graph));
+        hipGraphExec_t exec;
+        TD_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+        it = ctx->dp_graphs.emplace(key, exec).first;
+        p = {sig[0], sig[1]};
+      }
+      TD_HIP(hipGraphLaunch(it->second, ctx->stream));
+      *launches += 1;
+    }
   } else if (mode == 1) {
     int maxnh = 0;
     for (auto& G : Gs) maxnh = std::max(maxnh, static_cast<int>(G.nh));
@@ -702,7 +744,7 @@ void LaunchBatch(td_ctx* ctx, int mode, int ngraphs,
         const uint32_t ntiles = static_cast<uint32_t>(Binom(Gs[i].nh, h));
         const int grid = GridFor(ctx, (ntiles + R - 1) / R, block);
         hipLaunchKernelGGL(DPLevelKernel<LB>, dim3(grid), dim3(block), 0,
-                           ctx->stream, dGs + i, cb, arena, h, ntiles, R);
+                           ctx->stream, dGs + i, cb, arena, h, ntiles, R, 0);
         ++*launches;
       }
   }

@@ -187,13 +187,15 @@ def check_level_recurrence(g, prev, cur, k, samples, rng):
         assert int(cur[r]) == want, (k, r)
 
 
-@pytest.mark.parametrize("which", [0, 4])
-def test_n30_properties(dp, oracle, which):
+@pytest.mark.parametrize("which,remap", [(0, 1), (4, 1), (1, 0)])
+def test_n30_properties(ctx, dp, oracle, which, remap):
     """Full-size: certified tree, depth = td, and the recurrence holds on
     sampled subsets of the middle levels read back from the device."""
     g = G.workload_dp_n30()[which]
-    print("n30", g.name, flush=True)
+    print("n30", g.name, remap, flush=True)
+    ctx.set_param("dp.xcd_remap", remap)
     dp(g)
+    ctx.set_param("dp.xcd_remap", 0)
     assert oracle.certify(g, [int(p) for p in dp.parent], dp.treedepth)
     rng = random.Random(99)
     for k in (2, 14, 15, 29, 30):
