@@ -34,6 +34,9 @@ def sources():
     return sorted(str(p) for p in CSRC.glob("*.hip"))
 
 
+CLI = HERE / "examples" / "solve_cli"
+
+
 def build_library(verbose: bool = True) -> Path:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = LIB.with_suffix(".so.tmp")
@@ -42,7 +45,19 @@ def build_library(verbose: bool = True) -> Path:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
+
+
+def build_cli(verbose: bool = True) -> Path:
+    """The C++ example caller (examples/solve_cli.cpp): plain g++ against
+    include/tdhip.hpp, linked to libtdhip.so with an rpath to this folder."""
+    cmd = ["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", str(CLI) + ".cpp", "-o", str(CLI),
+           f"-L{HERE}", "-ltdhip", f"-Wl,-rpath,{HERE}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -1,0 +1,172 @@
+// tdhip.hpp -- C++ host adapters over the C ABI of libtdhip.so, with the
+// method names of the reference's classes so callers of
+//   td::DynamicGPU  (Treedepth/src/dynamic_gpu/dynamic_gpu.hpp:15-64)
+//   td::BnBGPU      (Treedepth/src/bnb_gpu/bnb_gpu.hpp:8-31)
+//   td::EliminationTree::Result (Treedepth/src/elimination_tree/
+//                                elimination_tree.hpp:26-30)
+// switch by changing the graph type they pass.  Boost is not a dependency:
+// a graph is td::Graph (n + bitmask rows), with AddEdge like boost::add_edge.
+// Errors become td::Error exceptions on this side of the boundary only.
+#ifndef TDHIP_HPP_
+#define TDHIP_HPP_
+
+#include <cstdint>
+#include <list>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "tdhip.h"
+
+namespace td {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// Undirected simple graph on vertices 0..n-1 (n <= 64).
+struct Graph {
+  int n = 0;
+  std::vector<uint64_t> adj;
+  explicit Graph(int nverts = 0) : n(nverts), adj(nverts, 0) {}
+  void AddEdge(int a, int b) {
+    if (a == b) throw Error(TD_E_INVALID, "Self loops are not allowed");
+    adj.at(a) |= 1ull << b;
+    adj.at(b) |= 1ull << a;
+  }
+  std::size_t NumEdges() const {
+    std::size_t e = 0;
+    for (uint64_t r : adj) e += static_cast<std::size_t>(__builtin_popcountll(r));
+    return e / 2;
+  }
+};
+
+// A decomposition as the reference returns it: tree edges, treedepth, root.
+// parent[v] = -1 for a root (several roots for a disconnected input).
+struct Result {
+  std::vector<std::pair<int, int>> td_decomp;
+  unsigned treedepth = 0;
+  std::size_t root = 0;
+  std::vector<int> parent;
+  static Result FromParent(const std::vector<int>& parent, unsigned treedepth) {
+    Result r;
+    r.parent = parent;
+    r.treedepth = treedepth;
+    for (std::size_t v = 0; v < parent.size(); ++v) {
+      if (parent[v] >= 0) r.td_decomp.emplace_back(static_cast<int>(v), parent[v]);
+    }
+    std::size_t u = 0;
+    while (!parent.empty() && parent[u] >= 0) u = static_cast<std::size_t>(parent[u]);
+    r.root = u;
+    return r;
+  }
+};
+
+// Owns a td_ctx (one per host thread).
+class Context {
+ public:
+  explicit Context(int device = 0, uint64_t mem_budget_bytes = 0) {
+    td_ctx* c = nullptr;
+    const int rc = td_ctx_create(&c, device, mem_budget_bytes);
+    if (rc != TD_OK) throw Error(rc, "td_ctx_create failed");
+    ctx_.reset(c);
+  }
+  td_ctx* get() const { return ctx_.get(); }
+  void Check(int rc) const {
+    if (rc != TD_OK) throw Error(rc, td_last_error(ctx_.get()));
+  }
+  void SetParam(const char* key, int64_t v) { Check(td_ctx_set_param(get(), key, v)); }
+
+ private:
+  struct Del {
+    void operator()(td_ctx* c) const { td_ctx_destroy(c); }
+  };
+  std::unique_ptr<td_ctx, Del> ctx_;
+};
+
+// td::DynamicGPU: operator()(g) runs the subset DP; the level tables stay on
+// the device for GetElimination / GetTreedepth.
+class DynamicGPU {
+ public:
+  explicit DynamicGPU(std::shared_ptr<Context> ctx = std::make_shared<Context>())
+      : ctx_(std::move(ctx)) {}
+
+  void operator()(const Graph& g) {
+    uint8_t td = 0;
+    std::vector<int8_t> par(static_cast<std::size_t>(g.n));
+    ctx_->Check(td_dp_solve(ctx_->get(), g.n, g.adj.data(), &td, par.data(), &stats_));
+    nverts_ = g.n;
+    treedepth_ = td;
+    parent_.assign(par.begin(), par.end());
+  }
+
+  template <typename VertexType = std::size_t>
+  std::list<VertexType> GetElimination(std::size_t nverts, std::size_t subset_size,
+                                       std::size_t subset_code) {
+    std::vector<int32_t> order(subset_size ? subset_size : 1);
+    ctx_->Check(td_dp_get_elimination(ctx_->get(), static_cast<int32_t>(nverts),
+                                      static_cast<int32_t>(subset_size), subset_code,
+                                      order.data()));
+    std::list<VertexType> out;
+    for (std::size_t i = 0; i < subset_size; ++i)
+      out.push_back(static_cast<VertexType>(order[i]));
+    return out;
+  }
+
+  // DynamicCPU::GetTreedepth counterpart (any subset, colex code).
+  std::size_t GetTreedepth(std::size_t /*nverts*/, std::size_t subset_size,
+                           std::size_t subset_code) const {
+    uint32_t v = 0;
+    ctx_->Check(td_dp_get_treedepth(ctx_->get(), static_cast<int32_t>(subset_size),
+                                    subset_code, &v));
+    return v;
+  }
+
+  // The decomposition the reference builds from GetElimination(n, n, 0)
+  // through EliminationTree::Decompose (Treedepth/main.cpp:187-195).
+  Result GetTDDecomp() const { return Result::FromParent(parent_, treedepth_); }
+  const td_dp_stats& Stats() const { return stats_; }
+
+ private:
+  std::shared_ptr<Context> ctx_;
+  int nverts_ = 0;
+  unsigned treedepth_ = 0;
+  std::vector<int> parent_;
+  td_dp_stats stats_{};
+};
+
+// td::BnBGPU: int operator()(g, heur_td) like the reference; Solve(g) also
+// returns the decomposition and whether the search completed.
+class BnBGPU {
+ public:
+  explicit BnBGPU(std::shared_ptr<Context> ctx = std::make_shared<Context>())
+      : ctx_(std::move(ctx)) {}
+
+  int operator()(const Graph& g, std::size_t heur_td) {
+    int32_t td = 0;
+    uint8_t cert = 0;
+    std::vector<int8_t> par(static_cast<std::size_t>(g.n));
+    ctx_->Check(td_bnb_solve(ctx_->get(), g.n, g.adj.data(), static_cast<int32_t>(heur_td),
+                             &td, par.data(), &cert));
+    parent_.assign(par.begin(), par.end());
+    certified_ = cert != 0;
+    return td;
+  }
+
+  std::pair<Result, bool> Solve(const Graph& g) {
+    const int td = (*this)(g, 0);
+    return {Result::FromParent(parent_, static_cast<unsigned>(td)), certified_};
+  }
+
+ private:
+  std::shared_ptr<Context> ctx_;
+  std::vector<int> parent_;
+  bool certified_ = false;
+};
+
+}  // namespace td
+
+#endif  // TDHIP_HPP_

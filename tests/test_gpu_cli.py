@@ -1,0 +1,33 @@
+"""The C++ adapters (include/tdhip.hpp) through the example CLI, on the
+GPU: the reference's P3 known answer (td 2, root 1) and a cycle."""
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from gisbdw_amd import graphs as G
+
+pytestmark = pytest.mark.gpu
+CLI = Path(__file__).resolve().parent.parent / "gisbdw_amd" / "examples" / "solve_cli"
+
+
+def run(alg, g, tmp_path):
+    f = tmp_path / "g.gviz"
+    f.write_text(G.format_graph_text(g))
+    r = subprocess.run([str(CLI), "-a", alg, str(f)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = dict(line.split(": ") for line in r.stdout.splitlines() if ": " in line)
+    edges = [tuple(map(int, line.rstrip(";").split(" -- "))) for line in r.stdout.splitlines()
+             if " -- " in line]
+    return out, edges
+
+
+@pytest.mark.parametrize("alg", ["dynGPU", "bnbGPU"])
+def test_cli_p3_and_cycle(alg, tmp_path, oracle):
+    out, edges = run(alg, G.path(3), tmp_path)
+    assert out["Treedepth"] == "2" and out["Root"] == "1" and out["Certified"] == "1"
+    assert sorted(tuple(sorted(e)) for e in edges) == [(0, 1), (1, 2)]
+    g = G.cycle(11)
+    out, edges = run(alg, g, tmp_path)
+    assert int(out["Treedepth"]) == oracle.dp_fast(g)[0]

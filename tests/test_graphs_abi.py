@@ -93,3 +93,20 @@ def test_product_path_fails_loudly_without_gpu():
         pytest.skip("a GPU may be present")
     with pytest.raises(api.TDError):
         api.Context()
+
+
+def test_cpp_cli_built_and_fails_loudly_without_gpu(tmp_path):
+    """The C++ caller over include/tdhip.hpp exists; with no device it
+    reports the HIP error instead of computing anything on the CPU."""
+    import os
+    import subprocess
+
+    cli = ROOT / "gisbdw_amd" / "examples" / "solve_cli"
+    if not cli.exists():
+        pytest.skip("solve_cli not built")
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU may be present")
+    f = tmp_path / "p3.gviz"
+    f.write_text("graph {\n0;\n1;\n2;\n0 -- 1;\n1 -- 2;\n}\n")
+    r = subprocess.run([str(cli), "-a", "dynGPU", str(f)], capture_output=True, text=True)
+    assert r.returncode == 1 and "error 3" in r.stderr
