@@ -197,39 +197,60 @@ __device__ __forceinline__ uint32_t BucketOf(uint64_t C, uint32_t mask) {
   return static_cast<uint32_t>((C * 0x9E3779B97F4A7C15ull) >> 40) & mask;
 }
 
-// Probe in two halves so the load's latency overlaps other work: MemoIssue
-// loads C's bucket (lanes 0-3, one entry each; the workgroup barrier first
-// retires this wave's earlier entry stores); MemoResolve returns the entry's
-// data (0 if absent) and its slot, or -1-victim when absent.
-__device__ __forceinline__ uint4 MemoIssue(const Memo& m, uint64_t C) {
-  Sync();
-  const int lane = threadIdx.x & 63;
-  uint4 e = make_uint4(0, 0, 0, 0);
-  if (lane < 4) e = m.table[static_cast<size_t>(BucketOf(C, m.mask)) * 4 + lane];
-  return e;
+// The memo is read and written by lane 0 only and its results are taken
+// from lane 0 (readfirstlane), so one thread's program order keeps every
+// probe after every earlier store of the wave: no barrier is needed.
+// MemoIssue loads C's bucket (4 entries, 64 bytes) so the latency overlaps
+// other work; MemoResolve returns the entry's data (0 if absent) and its
+// slot, or -1-victim when absent.
+struct Bucket {
+  uint4 e[4];
+};
+
+__device__ __forceinline__ Bucket MemoIssue(const Memo& m, uint64_t C) {
+  Bucket b;
+  if ((threadIdx.x & 63) == 0) {
+    const uint4* p = m.table + static_cast<size_t>(BucketOf(C, m.mask)) * 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b.e[i] = p[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b.e[i] = make_uint4(0, 0, 0, 0);
+  }
+  return b;
 }
 
 __device__ __forceinline__ uint32_t MemoResolve(const Memo& m, uint64_t C,
-                                                uint4 e, int* slot) {
-  const int lane = threadIdx.x & 63;
-  const bool hit = lane < 4 && e.w == m.tag && e.x == static_cast<uint32_t>(C) &&
-                   e.y == static_cast<uint32_t>(C >> 32);
-  const uint64_t hb = Ballot(hit);
-  if (hb) {
-    const int s = __builtin_ctzll(hb);
-    *slot = s;
-    return static_cast<uint32_t>(__shfl(static_cast<int>(e.z), s));
+                                                const Bucket& b, int* slot) {
+  int s = -1, free_s = -1;
+  uint32_t data = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint4 e = b.e[i];
+    const bool hit = e.w == m.tag && e.x == static_cast<uint32_t>(C) &&
+                     e.y == static_cast<uint32_t>(C >> 32);
+    if (hit && s < 0) {
+      s = i;
+      data = e.z;
+    }
+    if (e.w != m.tag && free_s < 0) free_s = i;
   }
-  const uint64_t fb = Ballot(lane < 4 && e.w != m.tag);
-  *slot = fb ? -1 - __builtin_ctzll(fb)
-             : -1 - static_cast<int>((C >> 40) & 3ull);  // victim
+  // lane 0's view decides
+  s = Uniform32(s);
+  free_s = Uniform32(free_s);
+  data = static_cast<uint32_t>(Uniform32(static_cast<int>(data)));
+  if (s >= 0) {
+    *slot = s;
+    return data;
+  }
+  *slot = free_s >= 0 ? -1 - free_s : -1 - static_cast<int>((C >> 40) & 3ull);  // victim
   return 0u;
 }
 
 __device__ __forceinline__ uint32_t MemoProbe(const Memo& m, uint64_t C,
                                               int* slot) {
-  const uint4 e = MemoIssue(m, C);
-  return MemoResolve(m, C, e, slot);
+  const Bucket b = MemoIssue(m, C);
+  return MemoResolve(m, C, b, slot);
 }
 
 __device__ __forceinline__ void MemoStore(const Memo& m, uint64_t C, int slot,
@@ -268,14 +289,53 @@ __device__ __forceinline__ uint64_t RootKey(bool use_scores, float score,
   return (static_cast<uint64_t>(primary) << 8) | static_cast<uint64_t>(63 - lane);
 }
 
+// Highest key among cand, ties to the lowest vertex: one ballot per key bit
+// from the top, keeping the lanes that have the bit whenever any does.
 __device__ __forceinline__ int PickRoot(bool use_scores, float score,
                                         uint64_t myadj, uint64_t C,
                                         uint64_t cand) {
   const int lane = threadIdx.x & 63;
-  const bool in = (cand >> lane) & 1ull;
-  const uint64_t key = in ? RootKey(use_scores, score, myadj, C, lane) + 1 : 0;
-  const uint64_t best = WaveMax64(key);
-  return __builtin_ctzll(Ballot(in && key == best));
+  uint32_t key;
+  int bits;
+  if (use_scores) {
+    const uint32_t u = __float_as_uint(score);
+    key = ((u & 0x80000000u) ? ~u : (u | 0x80000000u)) >> 16;  // order-preserving
+    bits = 16;
+  } else {
+    key = static_cast<uint32_t>(__builtin_popcountll(myadj & C));
+    bits = 7;
+  }
+  uint64_t M = cand;
+  for (int b = bits - 1; b >= 0; --b) {
+    const uint64_t t = Ballot(((M >> lane) & 1ull) && ((key >> b) & 1u));
+    if (t) M = t;
+  }
+  return __builtin_ctzll(M);
+}
+
+// Roots worth trying in C.  Root v dominates root u when
+// N_C(u) \ {v} is a subset of N_C(v) \ {u}: then C \ v is isomorphic to a
+// subgraph of C \ u (u -> v), so td(C \ v) <= td(C \ u) and u may be
+// skipped.  Domination implies deg_C(u) <= deg_C(v) with equality only for
+// twins; u is skipped only for a dominator of higher degree, or of equal
+// degree and lower id, so every chain of skips ends at a root that is tried.
+__device__ __forceinline__ uint64_t Undominated(uint64_t myadj, uint64_t C) {
+  const int lane = threadIdx.x & 63;
+  const bool in = (C >> lane) & 1ull;
+  const uint64_t Nu = myadj & C;
+  const int du = __builtin_popcountll(Nu);
+  bool dominated = false;
+  for (uint64_t m = C; m; m &= m - 1) {
+    const int v = __builtin_ctzll(m);
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(myadj), v);
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(myadj >> 32), v);
+    const uint64_t Nv = ((static_cast<uint64_t>(hi) << 32) | lo) & C;
+    const int dv = __builtin_popcountll(Nv);
+    const bool sub = ((Nu & ~(1ull << v)) & ~(Nv & ~(1ull << lane))) == 0ull;
+    const bool higher = dv > du || (dv == du && v < lane);
+    dominated |= in && v != lane && sub && higher;
+  }
+  return C & ~Ballot(dominated);
 }
 
 struct Search {
@@ -288,7 +348,9 @@ struct Search {
   bool aborted;
 };
 
-// Decision search td(C0) <= k0 ?  Uniform across the wave.
+// Decision search td(C0) <= k0 ?  Uniform across the wave.  Stack frames
+// are written by lane 0 and read back through readfirstlane (lane 0's own
+// view), so no barrier is needed between a frame update and its use.
 __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
   const int lane = threadIdx.x & 63;
   int sp = 0;
@@ -305,7 +367,6 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
       S.st[sp] = f;
     }
     ++sp;
-    Sync();
   };
   push(C0, k0);
   while (sp > 0) {
@@ -326,7 +387,7 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
         continue;
       }
       // the memo load is in flight while the bounds are computed
-      const uint4 me = MemoIssue(S.memo, C);
+      const Bucket me = MemoIssue(S.memo, C);
       const bool in_c = (C >> lane) & 1ull;
       const int lb = LowerBound(S.myadj, in_c, C, k);
       int slot;
@@ -342,11 +403,11 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
         --sp;
         continue;
       }
+      const uint64_t roots = n >= 4 ? Undominated(S.myadj, C) : C;
       if (lane == 0) {
-        F->cand = C;
+        F->cand = roots;
         F->state = kNextRoot;
       }
-      Sync();
       continue;
     }
     if (state == kNextRoot) {
@@ -368,7 +429,6 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
         F->pending = C & ~(1ull << v);
         F->state = kNextComp;
       }
-      Sync();
       continue;
     }
     if (state == kNextComp) {
@@ -411,12 +471,10 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
       }
       if (reject) {
         if (lane == 0) F->state = kNextRoot;
-        Sync();
         continue;
       }
       if (!D) {  // only trivial components left
         if (lane == 0) F->pending = 0;
-        Sync();
         continue;
       }
       if (lane == 0) {
@@ -424,7 +482,6 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
         F->curD = D;
         F->state = kAfterChild;
       }
-      Sync();
       if (sp >= kStack) {  // cannot happen: depth <= 2 * 64 + 1
         S.aborted = true;
         ret = false;
@@ -443,7 +500,6 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
         F->state = kNextRoot;
       }
     }
-    Sync();
   }
   return ret;
 }
