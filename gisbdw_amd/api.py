@@ -105,6 +105,8 @@ def load_library():
                                          P(C.c_uint8), P(BnBStats)]),
         "td_bnb_solve": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_uint64), C.c_int32,
                                    P(C.c_int32), P(C.c_int8), P(C.c_uint8)]),
+        "td_bnb_last_details": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int32),
+                                          P(C.c_uint64)]),
     })
     sig.update({
         "td_mlp_num_params": (C.c_int, []),
@@ -349,3 +351,14 @@ def bnb_solve_batch(ctx: Context, graphs: Sequence[Graph], ub_seed=None, scores=
         parents.append(par[off: off + g.n].astype(int).tolist())
         off += g.n
     return td.astype(int).tolist(), parents, cert.astype(bool).tolist(), st
+
+
+def bnb_last_details(ctx: Context, count: int):
+    """(heuristic tree depth, proven lower bound, nodes) per graph of the last
+    branch-and-bound call."""
+    heur = np.zeros(max(count, 1), np.int32)
+    lb = np.zeros(max(count, 1), np.int32)
+    nodes = np.zeros(max(count, 1), np.uint64)
+    ctx.check(ctx.lib.td_bnb_last_details(ctx.h, count, _ptr(heur, C.c_int32),
+                                          _ptr(lb, C.c_int32), _ptr(nodes, C.c_uint64)))
+    return heur[:count], lb[:count], nodes[:count]

@@ -33,7 +33,7 @@ namespace tdhip {
 namespace {
 
 constexpr int kMaxBnBN = 64;
-constexpr int kStack = 160;
+constexpr int kStack = 72;  // Feasible(C, k) nests at most k + 1 <= 65 frames
 
 struct BnBGraph {
   uint32_t n;
@@ -61,9 +61,15 @@ struct Frame {
 
 enum : int { kEnter = 0, kNextRoot = 1, kNextComp = 2, kAfterChild = 3 };
 
-// One wave per workgroup: a workgroup barrier orders lane 0's LDS and
-// global writes before every lane's later reads.
-__device__ __forceinline__ void Sync() { __syncthreads(); }
+// Orders this wave's LDS writes before its later reads by other lanes of
+// the same wave (LDS operations of one wave execute in order; the fence
+// keeps the compiler from moving them).  Workgroup-wide points use
+// __syncthreads() explicitly.
+__device__ __forceinline__ void Sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ __forceinline__ uint64_t Ballot(bool p) {
   return static_cast<uint64_t>(__ballot(p));
@@ -345,8 +351,23 @@ struct Search {
   bool use_scores;
   Memo memo;
   uint64_t nodes, budget, hits;
-  bool aborted;
+  bool aborted;          // node budget exhausted: results are not proofs
+  const int* cancel;     // LDS flag set when another wave of the graph succeeded
 };
+
+// A halted search (budget or cancellation) returns false without storing a
+// lower bound, so the memo only ever holds proven facts.
+__device__ __forceinline__ bool Halted(Search& S) {
+  if (S.aborted) return true;
+  if (S.cancel) {
+    const int c = __atomic_load_n(S.cancel, __ATOMIC_RELAXED);
+    if (Uniform32(c)) {
+      S.aborted = true;
+      return true;
+    }
+  }
+  return false;
+}
 
 // Decision search td(C0) <= k0 ?  Uniform across the wave.  Stack frames
 // are written by lane 0 and read back through readfirstlane (lane 0's own
@@ -378,7 +399,7 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
       const int n = __builtin_popcountll(C);
       if (n <= k) { ret = true; --sp; continue; }
       if (k <= 0) { ret = false; --sp; continue; }
-      if (S.aborted) { ret = false; --sp; continue; }
+      if (Halted(S)) { ret = false; --sp; continue; }
       ++S.nodes;
       if (S.budget && S.nodes > S.budget) {
         S.aborted = true;
@@ -660,26 +681,39 @@ __device__ int MinDegreeTree(uint64_t myadj, uint64_t V, int8_t* parent,
   return static_cast<int>(mx);
 }
 
-__global__ __launch_bounds__(64) void BnBKernel(
+// One workgroup of W waves per graph, graphs taken from a device queue.
+// Wave 0 computes the upper-bound trees and the static bound; then for each
+// k = UB-1, UB-2, ... the W waves split the top-level root candidates (in key
+// order, candidate i to wave i mod W), each searching its candidates with a
+// private memo.  A wave that finds a decomposition raises s_found and the
+// others stop; if none does and none ran out of budget, td > k is proven.
+constexpr int kMaxWaves = 4;
+
+__global__ __launch_bounds__(256) void BnBKernel(
     const BnBGraph* __restrict__ gs, BnBOut* __restrict__ outs,
     uint4* __restrict__ memo_base, int memo_log2, uint64_t node_budget,
     int ngraphs, uint32_t* __restrict__ queue, uint32_t tag_base) {
-  __shared__ Frame s_stack[kStack];
+  __shared__ Frame s_stack[kMaxWaves][kStack];
   __shared__ int8_t s_parent[64], s_best[64];
   __shared__ uint64_t s_bst[130];
   __shared__ int s_bk[130], s_bpar[130];
-  __shared__ int s_gi;
   __shared__ uint64_t s_comps[64];
+  __shared__ int s_gi, s_ub, s_lb, s_heur, s_ncomp;
+  __shared__ int s_found, s_winner, s_wroot, s_abort;
+  __shared__ unsigned long long s_nodes, s_hits;
   const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int W = blockDim.x >> 6;
   Search S;
-  S.st = s_stack;
-  S.memo.table = memo_base + (static_cast<size_t>(blockIdx.x) << (memo_log2 + 2));
+  S.st = s_stack[wid];
+  S.memo.table =
+      memo_base + (static_cast<size_t>(blockIdx.x * W + wid) << (memo_log2 + 2));
   S.memo.mask = (1u << memo_log2) - 1u;
   for (;;) {
-    if (lane == 0) s_gi = static_cast<int>(atomicAdd(queue, 1u));
-    Sync();
-    const int gi = Uniform32(s_gi);
-    if (gi >= ngraphs) break;
+    if (threadIdx.x == 0) s_gi = static_cast<int>(atomicAdd(queue, 1u));
+    __syncthreads();
+    const int gi = s_gi;
+    if (gi >= ngraphs) break;  // uniform across the workgroup
     const BnBGraph* g = gs + gi;
     const int n = static_cast<int>(g->n);
     const uint64_t V = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
@@ -689,95 +723,199 @@ __global__ __launch_bounds__(64) void BnBKernel(
     S.memo.tag = tag_base + static_cast<uint32_t>(gi) + 1u;
     S.nodes = 0;
     S.hits = 0;
-    S.budget = node_budget;
+    // the node budget is per graph, shared evenly by its waves
+    S.budget = node_budget ? max(node_budget / static_cast<uint64_t>(W), 1ull) : 0;
     S.aborted = false;
-    // ---- initial upper bound: best of the greedy trees --------------------
-    int ub = GreedyTree(S, V, false, s_best, s_bst, s_bk, s_bpar);
-    if (S.use_scores) {
-      const int t = GreedyTree(S, V, true, s_parent, s_bst, s_bk, s_bpar);
-      if (t < ub) {
-        ub = t;
-        if (lane < 64) s_best[lane] = s_parent[lane];
-      }
+    S.cancel = nullptr;
+    if (threadIdx.x == 0) {
+      s_nodes = 0;
+      s_hits = 0;
+      s_abort = 0;
     }
-    {
-      const int t = MinDegreeTree(S.myadj, V, s_parent, nullptr);
-      if (t < ub) {
-        ub = t;
-        s_best[lane] = s_parent[lane];
-      }
-    }
-    Sync();
-    const int heur = ub;
-    // components of the input
-    uint64_t* comps = s_comps;
-    int ncomp = 0;
-    {
-      uint64_t rest = V;
-      while (rest) {
-        const uint64_t D = Component(S.myadj, rest, rest & (0ull - rest), nullptr, nullptr);
-        if (lane == 0) comps[ncomp] = D;
-        ++ncomp;
-        rest &= ~D;
-      }
-      Sync();
-    }
-    // caller's seed: accept it only with a tree of that depth
-    const int seed = g->ub_seed;
-    if (seed > 0 && seed < ub) {
-      bool ok = true;
-      for (int c = 0; c < ncomp && ok; ++c) ok = Feasible(S, Uniform64(comps[c]), seed);
-      if (ok && !S.aborted) {
-        s_parent[lane] = -1;
-        Sync();
-        for (int c = 0; c < ncomp && !S.aborted; ++c)
-          Build(S, Uniform64(comps[c]), seed, -1, s_parent, s_bst, s_bk, s_bpar);
-        if (!S.aborted) {
-          ub = seed;
+    if (wid == 0) {
+      // ---- initial upper bound: best of the greedy trees ----------------
+      int ub = GreedyTree(S, V, false, s_best, s_bst, s_bk, s_bpar);
+      if (S.use_scores) {
+        const int t = GreedyTree(S, V, true, s_parent, s_bst, s_bk, s_bpar);
+        if (t < ub) {
+          ub = t;
           s_best[lane] = s_parent[lane];
-          Sync();
         }
       }
+      {
+        const int t = MinDegreeTree(S.myadj, V, s_parent, nullptr);
+        if (t < ub) {
+          ub = t;
+          s_best[lane] = s_parent[lane];
+        }
+      }
+      Sync();
+      const int heur = ub;
+      int ncomp = 0;
+      {
+        uint64_t rest = V;
+        while (rest) {
+          const uint64_t D = Component(S.myadj, rest, rest & (0ull - rest), nullptr, nullptr);
+          if (lane == 0) s_comps[ncomp] = D;
+          ++ncomp;
+          rest &= ~D;
+        }
+      }
+      // caller's seed: accept it only with a tree of that depth
+      const int seed = g->ub_seed;
+      if (seed > 0 && seed < ub) {
+        bool ok = true;
+        for (int c = 0; c < ncomp && ok; ++c) ok = Feasible(S, Uniform64(s_comps[c]), seed);
+        if (ok && !S.aborted) {
+          s_parent[lane] = -1;
+          Sync();
+          for (int c = 0; c < ncomp && !S.aborted; ++c)
+            Build(S, Uniform64(s_comps[c]), seed, -1, s_parent, s_bst, s_bk, s_bpar);
+          if (!S.aborted) {
+            ub = seed;
+            s_best[lane] = s_parent[lane];
+            Sync();
+          }
+        }
+        S.aborted = false;  // the seed check does not decide certification
+      }
+      int lb = 1;
+      for (int c = 0; c < ncomp; ++c) {
+        const uint64_t Cc = Uniform64(s_comps[c]);
+        lb = max(lb, LowerBound(S.myadj, (Cc >> lane) & 1ull, Cc, 1 << 10));
+      }
+      lb = max(lb, Degeneracy(S.myadj, V) + 1);
+      if (lane == 0) {
+        s_ub = ub;
+        s_lb = lb;
+        s_heur = heur;
+        s_ncomp = ncomp;
+      }
     }
-    // static lower bound of the whole graph
-    int lb = 1;
-    for (int c = 0; c < ncomp; ++c) {
-      const uint64_t Cc = Uniform64(comps[c]);
-      const bool in_c = (Cc >> lane) & 1ull;
-      lb = max(lb, LowerBound(S.myadj, in_c, Cc, 1 << 10));
-    }
-    lb = max(lb, Degeneracy(S.myadj, V) + 1);
-    // ---- descend: try ub-1 until it fails --------------------------------
-    while (ub - 1 >= lb && !S.aborted) {
+    __syncthreads();
+    int ub = s_ub, lb = s_lb;
+    const int ncomp = s_ncomp;
+    bool uncertain = false;
+    // ---- descend: k = ub-1 until it fails -------------------------------
+    while (ub - 1 >= lb) {
       const int k = ub - 1;
-      bool ok = true;
-      for (int c = 0; c < ncomp && ok; ++c) ok = Feasible(S, Uniform64(comps[c]), k);
-      if (S.aborted) break;
-      if (!ok) {
-        lb = ub;
+      bool all_ok = true;
+      for (int c = 0; c < ncomp && all_ok; ++c) {
+        const uint64_t C = s_comps[c];
+        const int nc = __builtin_popcountll(C);
+        if (nc <= k) continue;
+        if (threadIdx.x == 0) {
+          s_found = 0;
+          s_winner = -1;
+          s_wroot = -1;
+        }
+        __syncthreads();
+        S.cancel = &s_found;
+        const bool in_c = (C >> lane) & 1ull;
+        const bool lb_fail = LowerBound(S.myadj, in_c, C, k) > k;
+        if (!lb_fail) {
+          // this wave's share of the top-level roots, in key order
+          uint64_t cand = nc >= 4 ? Undominated(S.myadj, C) : C;
+          for (int i = 0; cand; ++i) {
+            const int v = PickRoot(S.use_scores, S.score, S.myadj, C, cand);
+            cand &= ~(1ull << v);
+            if (i % W != wid) continue;
+            if (Halted(S)) break;
+            bool ok = true;
+            uint64_t rest = C & ~(1ull << v);
+            while (rest && ok) {
+              const uint64_t D = Component(S.myadj, rest, rest & (0ull - rest), nullptr, nullptr);
+              rest &= ~D;
+              if (__builtin_popcountll(D) > k - 1) ok = Feasible(S, D, k - 1);
+            }
+            if (ok && !S.aborted) {
+              if (lane == 0 && atomicCAS(&s_found, 0, 1) == 0) {
+                s_winner = wid;
+                s_wroot = v;
+              }
+              break;
+            }
+          }
+        }
+        if (S.aborted && !__atomic_load_n(&s_found, __ATOMIC_RELAXED) && lane == 0)
+          atomicOr(&s_abort, 1);
+        S.cancel = nullptr;
+        __syncthreads();
+        const bool found = s_found != 0;
+        if (!found) {
+          all_ok = false;
+          if (s_abort) uncertain = true;
+          break;
+        }
+        // the winning wave rebuilds this component's tree with its memo
+        if (wid == s_winner) {
+          const int v = s_wroot;
+          if (c == 0) s_parent[lane] = -1;
+          Sync();
+          if (lane == 0) s_parent[v] = -1;
+          Sync();
+          S.aborted = false;
+          uint64_t rest = C & ~(1ull << v);
+          while (rest && !S.aborted) {
+            const uint64_t D = Component(S.myadj, rest, rest & (0ull - rest), nullptr, nullptr);
+            rest &= ~D;
+            Build(S, D, k - 1, v, s_parent, s_bst, s_bk, s_bpar);
+          }
+          if (S.aborted && lane == 0) atomicOr(&s_abort, 2);
+        }
+        __syncthreads();
+        if (s_abort & 2) {  // no tree could be rebuilt: keep the previous best
+          all_ok = false;
+          uncertain = true;
+          break;
+        }
+        // reset per-wave halt state for the next component / k
+        S.aborted = false;
+        if (threadIdx.x == 0) s_abort = 0;
+        __syncthreads();
+      }
+      if (!all_ok) {
+        if (!uncertain) lb = ub;
         break;
       }
-      s_parent[lane] = -1;
-      Sync();
-      for (int c = 0; c < ncomp && !S.aborted; ++c)
-        Build(S, Uniform64(comps[c]), k, -1, s_parent, s_bst, s_bk, s_bpar);
-      if (S.aborted) break;
+      // components that fit in k are chained (path decomposition)
+      if (wid == 0) {
+        for (int c = 0; c < ncomp; ++c) {
+          const uint64_t C = Uniform64(s_comps[c]);
+          if (__builtin_popcountll(C) > k) continue;
+          int prev = -1;
+          for (uint64_t m = C; m; m &= m - 1) {
+            const int u = __builtin_ctzll(m);
+            if (lane == 0) s_parent[u] = static_cast<int8_t>(prev);
+            prev = u;
+          }
+        }
+        Sync();
+        s_best[lane] = s_parent[lane];
+      }
       ub = k;
-      s_best[lane] = s_parent[lane];
-      Sync();
+      __syncthreads();
     }
     if (ub - 1 < lb) lb = ub;
-    BnBOut* o = outs + gi;
-    if (lane < n) o->parent[lane] = s_best[lane];
+    // totals over the waves
     if (lane == 0) {
-      o->td = ub;
-      o->lb = lb;
-      o->certified = (!S.aborted && lb == ub) ? 1u : 0u;
-      o->heur_td = static_cast<uint32_t>(heur);
-      o->nodes = S.nodes;
-      o->memo_hits = S.hits;
+      atomicAdd(&s_nodes, static_cast<unsigned long long>(S.nodes));
+      atomicAdd(&s_hits, static_cast<unsigned long long>(S.hits));
     }
-    Sync();
+    __syncthreads();
+    BnBOut* o = outs + gi;
+    if (wid == 0) {
+      if (lane < n) o->parent[lane] = s_best[lane];
+      if (lane == 0) {
+        o->td = ub;
+        o->lb = lb;
+        o->certified = (!uncertain && lb == ub) ? 1u : 0u;
+        o->heur_td = static_cast<uint32_t>(s_heur);
+        o->nodes = s_nodes;
+        o->memo_hits = s_hits;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -830,13 +968,16 @@ void SolveBnB(td_ctx* ctx, int ngraphs, const int32_t* ns, const uint64_t* adj_c
       G.score[v] = scores ? scores[off[i] + v] : 0.f;
     }
   }
-  const int memo_log2 = static_cast<int>(Param(ctx, "bnb.memo_log2", 19));
+  const int memo_log2 = static_cast<int>(Param(ctx, "bnb.memo_log2", 17));
   TD_REQUIRE(memo_log2 >= 8 && memo_log2 <= 24, TD_E_INVALID, "bnb.memo_log2 in [8, 24]");
   const uint64_t budget = static_cast<uint64_t>(Param(ctx, "bnb.node_budget", 2000000));
-  const int waves_cap = static_cast<int>(
-      Param(ctx, "bnb.waves", static_cast<int64_t>(ctx->prop.multiProcessorCount) * 4));
-  const int waves = std::max(1, std::min(ngraphs, waves_cap));
-  const size_t memo_bytes = (static_cast<size_t>(waves) << (memo_log2 + 2)) * sizeof(uint4);
+  const int W = static_cast<int>(Param(ctx, "bnb.waves_per_graph", 4));
+  TD_REQUIRE(W >= 1 && W <= kMaxWaves, TD_E_INVALID, "bnb.waves_per_graph in [1, 4]");
+  const int groups_cap = static_cast<int>(
+      Param(ctx, "bnb.groups", static_cast<int64_t>(ctx->prop.multiProcessorCount) * 4));
+  const int groups = std::max(1, std::min(ngraphs, groups_cap));
+  const size_t waves = static_cast<size_t>(groups) * W;
+  const size_t memo_bytes = (waves << (memo_log2 + 2)) * sizeof(uint4);
   auto* dG = static_cast<BnBGraph*>(ctx->Buffer("bnb.graphs", sizeof(BnBGraph) * ngraphs));
   auto* dO = static_cast<BnBOut*>(ctx->Buffer("bnb.out", sizeof(BnBOut) * ngraphs));
   auto* dM = static_cast<uint4*>(ctx->Buffer("bnb.memo", memo_bytes));
@@ -856,7 +997,7 @@ void SolveBnB(td_ctx* ctx, int ngraphs, const int32_t* ns, const uint64_t* adj_c
                         ctx->stream));
   TD_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
   hipLaunchKernelGGL(BnBResetQueue, dim3(1), dim3(64), 0, ctx->stream, dQ);
-  hipLaunchKernelGGL(BnBKernel, dim3(waves), dim3(64), 0, ctx->stream, dG, dO, dM,
+  hipLaunchKernelGGL(BnBKernel, dim3(groups), dim3(64 * W), 0, ctx->stream, dG, dO, dM,
                      memo_log2, budget, ngraphs, dQ, tag_base);
   TD_HIP(hipGetLastError());
   TD_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
@@ -864,8 +1005,10 @@ void SolveBnB(td_ctx* ctx, int ngraphs, const int32_t* ns, const uint64_t* adj_c
   TD_HIP(hipMemcpyAsync(ho.data(), dO, sizeof(BnBOut) * ngraphs, hipMemcpyDeviceToHost,
                         ctx->stream));
   TD_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->bnb_last.assign(ngraphs, {0, 0, 0});
   uint64_t cert = 0, nodes = 0, hits = 0;
   for (int i = 0; i < ngraphs; ++i) {
+    ctx->bnb_last[i] = {ho[i].heur_td, ho[i].lb, ho[i].nodes};
     if (td_out) td_out[i] = static_cast<uint8_t>(ho[i].td);
     if (certified_out) certified_out[i] = static_cast<uint8_t>(ho[i].certified);
     if (parent_concat) std::memcpy(parent_concat + off[i], ho[i].parent, ns[i]);
@@ -902,6 +1045,19 @@ int td_bnb_solve_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
   return Guard(ctx, [&] {
     SolveBnB(ctx, ngraphs, n, adj_concat, ub_seed, vertex_scores, td_out, parent_concat,
              certified_out, stats);
+  });
+}
+
+int td_bnb_last_details(td_ctx* ctx, int32_t count, int32_t* heur_td_out,
+                        int32_t* lower_bound_out, uint64_t* nodes_out) {
+  return Guard(ctx, [&] {
+    TD_REQUIRE(count >= 0 && static_cast<size_t>(count) <= ctx->bnb_last.size(), TD_E_STATE,
+               "count exceeds the graphs of the last branch-and-bound call");
+    for (int i = 0; i < count; ++i) {
+      if (heur_td_out) heur_td_out[i] = static_cast<int32_t>(ctx->bnb_last[i].heur);
+      if (lower_bound_out) lower_bound_out[i] = ctx->bnb_last[i].lb;
+      if (nodes_out) nodes_out[i] = ctx->bnb_last[i].nodes;
+    }
   });
 }
 

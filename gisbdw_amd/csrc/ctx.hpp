@@ -85,6 +85,12 @@ struct td_ctx {
   void* bnb_memo_ptr = nullptr;
   size_t bnb_memo_bytes = 0;
   uint32_t bnb_tag = 0;
+  struct BnBDetail {
+    uint32_t heur;
+    int32_t lb;
+    uint64_t nodes;
+  };
+  std::vector<BnBDetail> bnb_last;  // per graph, last td_bnb_* call
 
   // state of the last td_dp_solve (for the queries)
   int last_n = 0;

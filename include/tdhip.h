@@ -43,7 +43,8 @@ const char* td_last_error(const td_ctx* ctx);
 /* Tuning knobs (see td_dp_autotune): "dp.low_bits" (2..5, default 5),
  * "dp.tiles_per_lane" (1..64, default 1), "dp.block" (64..1024, default 256),
  * "dp.batch_wg_per_cu", "dp.use_graph" (0/1), "bnb.waves", "bnb.node_budget"
- * (default 2e6), "bnb.memo_log2" (default 19). */
+ * (per graph, default 2e6), "bnb.waves_per_graph" (1..4, default 4),
+ * "bnb.groups" (workgroups in flight), "bnb.memo_log2" (default 17). */
 int td_ctx_set_param(td_ctx* ctx, const char* key, int64_t value);
 int td_ctx_get_param(const td_ctx* ctx, const char* key, int64_t* value);
 
@@ -149,6 +150,13 @@ int td_bnb_solve_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
                        const float* vertex_scores, uint8_t* td_out,
                        int8_t* parent_concat, uint8_t* certified_out,
                        td_bnb_stats* stats);
+
+/* Per-graph details of the last td_bnb_solve / td_bnb_solve_batch call on
+ * this context (no reference counterpart; for analysis): the depth of the
+ * initial heuristic tree, the proven lower bound and the search nodes.
+ * Output arrays are nullable; count <= graphs of that call. */
+int td_bnb_last_details(td_ctx* ctx, int32_t count, int32_t* heur_td_out,
+                        int32_t* lower_bound_out, uint64_t* nodes_out);
 
 /* ---------------------------------------------------------------------- */
 /* Learned proposal (no reference counterpart: it feeds the reference's     */

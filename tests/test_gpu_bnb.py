@@ -107,3 +107,23 @@ def test_bnb_dense_exact_when_certified(ctx, oracle):
         assert t >= w
         if c:
             assert t == w, g.name
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_bnb_waves_per_graph_same_answers(ctx, oracle, waves):
+    rng = random.Random(90 + waves)
+    gs = [G.gnp(rng.randint(8, 22), rng.uniform(0.1, 0.35), rng.getrandbits(32)) for _ in range(20)]
+    gs += [G.tree_plus_edges(rng.randint(30, 44), 0.1, rng.getrandbits(32)) for _ in range(4)]
+    ctx.set_param("bnb.waves_per_graph", waves)
+    try:
+        tds, pars, cert, st = api.bnb_solve_batch(ctx, gs)
+    finally:
+        ctx.set_param("bnb.waves_per_graph", 4)
+    heur, lb, nodes = api.bnb_last_details(ctx, len(gs))
+    for i, (g, t, p, c) in enumerate(zip(gs, tds, pars, cert)):
+        assert oracle.certify(g, p, t)
+        assert heur[i] >= t and lb[i] <= t
+        if g.n <= 24:
+            assert c and t == oracle.dp_fast(g)[0], g.name
+        elif c:
+            assert lb[i] == t
