@@ -141,6 +141,72 @@ __device__ __forceinline__ int CeilLog2(int x) {  // x >= 1
   return x <= 1 ? 0 : 32 - __builtin_clz(static_cast<uint32_t>(x - 1));
 }
 
+// Treedepth of a tree, exactly: the optimal vertex ranking of Schaeffer
+// (1989), computed bottom-up over the BFS tree of C rooted at its lowest
+// vertex.  Each vertex keeps its critical set L(v) (bit r-1 = rank r is
+// visible from above v).  For v with children c_i:
+//   U = union of L(c_i), D = ranks present in two or more L(c_i),
+//   rank(v) = the smallest rank above max(D) that is not in U,
+//   L(v) = {rank(v)} u {x in U : x > rank(v)};
+// the largest rank is the treedepth of the tree and its vertex is unique.
+// For a connected C with |E(C)| = |C| - 1 the BFS tree is C: the result is
+// td(C) and *top an optimal root.  For any other connected C the BFS tree is
+// a spanning subgraph, so the result is a lower bound on td(C).
+// sL / sPar: this wave's LDS scratch (64 entries each).
+__device__ int TreeRank(uint64_t myadj, uint64_t C, int* top, uint64_t* sL, int8_t* sPar) {
+  const int lane = threadIdx.x & 63;
+  const bool in = (C >> lane) & 1ull;
+  const uint64_t root = C & (0ull - C);
+  int mylvl = (root >> lane) & 1ull ? 0 : -1;
+  uint64_t seen = root, front = root;
+  int depth = 0;
+  sPar[lane] = -1;
+  for (;;) {
+    const uint64_t nb = Ballot(in && !((seen >> lane) & 1ull) && (myadj & front) != 0ull);
+    if (!nb) break;
+    ++depth;
+    if ((nb >> lane) & 1ull) {
+      mylvl = depth;
+      sPar[lane] = static_cast<int8_t>(__builtin_ctzll(myadj & front));
+    }
+    seen |= nb;
+    front = nb;
+  }
+  Sync();
+  int rank = 0;
+  for (int l = depth; l >= 0; --l) {
+    const uint64_t below = Ballot(mylvl == l + 1);
+    if (mylvl == l) {
+      uint64_t U = 0, dup = 0;
+      for (uint64_t c = myadj & below; c; c &= c - 1) {
+        const int u = __builtin_ctzll(c);
+        if (sPar[u] == lane) {
+          const uint64_t Lu = sL[u];
+          dup |= U & Lu;
+          U |= Lu;
+        }
+      }
+      const int m = dup ? 63 - __builtin_clzll(dup) : -1;  // highest duplicated bit
+      const uint64_t above = m < 0 ? ~0ull : (m >= 63 ? 0ull : ~((2ull << m) - 1ull));
+      const uint64_t freeb = ~U & above;
+      const int b = freeb ? __builtin_ctzll(freeb) : 63;
+      rank = b + 1;
+      const uint64_t higher = b >= 63 ? 0ull : (U & ~((2ull << b) - 1ull));
+      sL[lane] = (1ull << b) | higher;
+    }
+    Sync();
+  }
+  // largest rank (<= 64, 7 bits) by ballots, and its (unique) vertex
+  uint64_t M = C;
+  for (int bit = 6; bit >= 0; --bit) {
+    const uint64_t t = Ballot(((M >> lane) & 1ull) && ((rank >> bit) & 1));
+    if (t) M = t;
+  }
+  const int v = __builtin_ctzll(M);
+  if (top) *top = v;
+  return __builtin_amdgcn_readlane(rank, v);
+}
+
 // Lower bound on td(C), C connected, all from proven facts.
 __device__ __forceinline__ int LowerBound(uint64_t myadj, bool in_c, uint64_t C,
                                           int k) {
@@ -353,6 +419,8 @@ struct Search {
   uint64_t nodes, budget, hits;
   bool aborted;          // node budget exhausted: results are not proofs
   const int* cancel;     // LDS flag set when another wave of the graph succeeded
+  uint64_t* tl;          // TreeRank scratch (this wave's LDS)
+  int8_t* tp;
 };
 
 // A halted search (budget or cancellation) returns false without storing a
@@ -410,7 +478,22 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
       // the memo load is in flight while the bounds are computed
       const Bucket me = MemoIssue(S.memo, C);
       const bool in_c = (C >> lane) & 1ull;
-      const int lb = LowerBound(S.myadj, in_c, C, k);
+      const int ecount =
+          static_cast<int>(WaveSum(in_c ? __builtin_popcountll(S.myadj & C) : 0)) / 2;
+      if (ecount == n - 1) {
+        // a tree: exact treedepth and an optimal root, no search
+        int top;
+        const int t = TreeRank(S.myadj, C, &top, S.tl, S.tp);
+        int slot;
+        const uint32_t md = MemoResolve(S.memo, C, me, &slot);
+        MemoStore(S.memo, C, slot, md, static_cast<uint32_t>(t), static_cast<uint32_t>(t),
+                  static_cast<uint32_t>(top));
+        ret = t <= k;
+        --sp;
+        continue;
+      }
+      int lb = LowerBound(S.myadj, in_c, C, k);
+      if (lb <= k && ecount - (n - 1) <= 12) lb = max(lb, TreeRank(S.myadj, C, nullptr, S.tl, S.tp));
       int slot;
       const uint32_t md = MemoResolve(S.memo, C, me, &slot);
       if (slot >= 0) {
@@ -484,6 +567,14 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
         if (EdgeLB(nd, ed) > k - 1) {
           reject = true;
           break;
+        }
+        if (ed == nd - 1) {  // a tree: decided exactly here
+          if (TreeRank(S.myadj, Dc, nullptr, S.tl, S.tp) > k - 1) {
+            reject = true;
+            break;
+          }
+          trivial |= Dc;
+          continue;
         }
         if (nd > bestn) {
           bestn = nd;
@@ -700,12 +791,16 @@ __global__ __launch_bounds__(256) void BnBKernel(
   __shared__ uint64_t s_comps[64];
   __shared__ int s_gi, s_ub, s_lb, s_heur, s_ncomp;
   __shared__ int s_found, s_winner, s_wroot, s_abort;
+  __shared__ uint64_t s_tl[kMaxWaves][64];
+  __shared__ int8_t s_tp[kMaxWaves][64];
   __shared__ unsigned long long s_nodes, s_hits;
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
   const int W = blockDim.x >> 6;
   Search S;
   S.st = s_stack[wid];
+  S.tl = s_tl[wid];
+  S.tp = s_tp[wid];
   S.memo.table =
       memo_base + (static_cast<size_t>(blockIdx.x * W + wid) << (memo_log2 + 2));
   S.memo.mask = (1u << memo_log2) - 1u;

@@ -127,3 +127,16 @@ def test_bnb_waves_per_graph_same_answers(ctx, oracle, waves):
             assert c and t == oracle.dp_fast(g)[0], g.name
         elif c:
             assert lb[i] == t
+
+
+def test_bnb_trees_exact(ctx, oracle):
+    """Tree components are decided by the optimal vertex ranking; random trees
+    and near-trees against the DP."""
+    rng = G.SplitMix64(5)
+    gs = [G.random_tree(rng.randbelow(28) + 2, rng) for _ in range(60)]
+    gs += [G.path(30), G.star(30), G.spanning_tree(G.grid(5, 6))]
+    gs += [G.tree_plus_edges(rng.randbelow(20) + 10, 0.1, rng.next_u64()) for _ in range(30)]
+    tds, pars, cert, st = api.bnb_solve_batch(ctx, gs)
+    for g, t, p, c in zip(gs, tds, pars, cert):
+        assert c and t == oracle.dp_fast(g)[0], g.name
+        assert oracle.certify(g, p, t)
