@@ -55,7 +55,7 @@ def parse():
     return ap.parse_args()
 
 
-def bnb_extra(ctx, budget=200000):
+def bnb_extra(ctx, budget=50000):
     """Branch-and-bound figures for the extra block: the 256-graph sparse
     workload with the plain search, then 1024 graphs of the same distribution
     plain vs learned (proposal trained here on the GPU on DP labels).
@@ -64,14 +64,25 @@ def bnb_extra(ctx, budget=200000):
     from gisbdw_amd import graphs as G
 
     ctx.set_param("bnb.node_budget", budget)
-    out = {"node_budget": budget}
+    ctx.set_param("bnb.waves_per_graph", 4)
+    out = {"node_budget_per_graph": budget, "waves_per_graph": 4}
+
+    def families(W, cert):
+        fam = {}
+        for g, c in zip(W, cert):
+            k = g.name.split("#")[0].rstrip("0123456789+-x.~")
+            k = "cubic" if k.startswith("cubic") else ("grid" if k.startswith("grid") else "tree+edges")
+            a = fam.setdefault(k, [0, 0])
+            a[0] += int(c)
+            a[1] += 1
+        return {k: f"{v[0]}/{v[1]}" for k, v in fam.items()}
     W = G.workload_bnb(256, 0)
     t0 = time.perf_counter()
     tds, pars, cert, st = api.bnb_solve_batch(ctx, W)
     wall = time.perf_counter() - t0
     out["bnb256_plain"] = {"certified": int(sum(cert)), "graphs": len(W), "wall_s": wall,
                            "certified_graphs_per_s": sum(cert) / wall, "nodes": st.nodes,
-                           "kernel_ms": st.kernel_ms}
+                           "kernel_ms": st.kernel_ms, "certified_by_family": families(W, cert)}
     prop = learn.Proposal(ctx)
     prop.init(1)
     t0 = time.perf_counter()
@@ -92,7 +103,8 @@ def bnb_extra(ctx, budget=200000):
     agree = all(a == b for a, b, c, d in zip(tds, tdl, cert, certl) if c and d)
     out["bnb1024"] = {
         "plain": {"certified": int(sum(cert)), "wall_s": wall_p,
-                  "certified_graphs_per_s": sum(cert) / wall_p, "nodes": st.nodes},
+                  "certified_graphs_per_s": sum(cert) / wall_p, "nodes": st.nodes,
+                  "certified_by_family": families(W, cert)},
         "learned": {"certified": int(sum(certl)), "wall_s": wall_l,
                     "certified_graphs_per_s": sum(certl) / wall_l, "nodes": stl.nodes,
                     "includes_scoring": True},
