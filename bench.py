@@ -12,9 +12,19 @@ achieved = algorithmic bytes / GPU time of the level launches (HIP events on
 the library's stream), algorithmic bytes = 2^(n-1) (n+2) per graph (k one-byte
 reads of level k-1 and one write per k-subset).
 
+Launch shape: td_dp_autotune picks (low_bits, tiles_per_lane, block) for n
+before the warm-up, unless --no-autotune or an explicit shape flag is given.
+
 cpu_baseline: the oracle's flat-table DP (OpenMP over each level) timed on the
 host on a bounded sample: the subgraph induced by the first m vertices of the
 first workload graph, m chosen so the run takes about 10-20 s.
+
+extra: copy bandwidth; branch-and-bound certified graphs/s on the 256- and
+1024-graph sparse workloads (plain and learned order, proposal trained here);
+the single G(20,0.3) graph end to end and the 4096-graph batch before and
+after td_dp_autotune_batch; and the oracle's CPU paths for the same
+configurations (faithful DynamicCPU, flat-table DP, BranchAndBound) on
+bounded samples.
 
 Other workloads: --workload dp20 | dpbatch.
 """
@@ -48,6 +58,8 @@ def parse():
     ap.add_argument("--xcd-remap", type=int, default=None, choices=[0, 1])
     ap.add_argument("--batch-block", type=int, default=None)
     ap.add_argument("--batch-wg-per-cu", type=int, default=None)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape instead of td_dp_autotune's pick")
     ap.add_argument("--no-bnb", action="store_true",
                     help="skip the branch-and-bound / learned-proposal extra figures")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -144,6 +156,113 @@ def cpu_baseline_dp(g, target_s=12.0):
                       f"{g.name}: 2^{m} subsets in {dt:.2f} s"}
 
 
+def gpu_configs_extra(ctx):
+    """GPU side of the single-graph (G(20,0.3) seed 0) and batched (4096
+    graphs, n in [12,24]) configurations, for comparison with cpu_extra."""
+    from gisbdw_amd import api
+    from gisbdw_amd import graphs as G
+
+    g = G.workload_dp_single(0)
+    dp = api.DynamicGPU(ctx)
+    dp(g)
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dp(g)
+    c1_ms = (time.perf_counter() - t0) / reps * 1e3
+    batch = G.workload_dp_batch(4096, 0)
+    ns, adj = api.pack_graphs(batch)
+    keys = ("dp.low_bits", "dp.tiles_per_lane", "dp.batch_block", "dp.batch_wg_per_cu")
+    saved = {k: ctx.get_param(k) for k in keys}
+
+    def timed():
+        api.dp_solve_batch_arrays(ctx, ns, adj, 0)
+        t0 = time.perf_counter()
+        api.dp_solve_batch_arrays(ctx, ns, adj, 0)
+        return time.perf_counter() - t0
+
+    c3_default = timed()
+    t0 = time.perf_counter()
+    api.dp_autotune_batch(ctx, batch)
+    tune_s = time.perf_counter() - t0
+    chosen = {k: ctx.get_param(k) for k in keys}
+    c3_tuned = timed()
+    for k, v in saved.items():
+        ctx.set_param(k, v)
+    return {"dp_n20_single": {"treedepth": dp.treedepth, "ms_per_graph_end_to_end": c1_ms},
+            "dp_batch4096": {"graphs_per_s_before_tuning": len(batch) / c3_default,
+                             "before_tuning": saved,
+                             "graphs_per_s_autotuned": len(batch) / c3_tuned,
+                             "autotuned": chosen, "autotune_s": tune_s}}
+
+
+def cpu_extra(threads):
+    """The oracle's CPU paths on bounded samples of the single-graph, batched
+    and branch-and-bound configurations, timed on this host:
+    - G(20,0.3) seed 0: the faithful DynamicCPU restatement (std::map per
+      level, dynamic_cpu.cpp:14-107) on one core over the subgraph induced by
+      the first m vertices (m picked for about 8 s), and the flat-table DP on
+      the whole graph with 1 and `threads` threads;
+    - 4096-graph batch: the flat-table DP, one graph per thread, on the first
+      256 graphs;
+    - 256 sparse graphs n in [40,64]: the BranchAndBound restatement with
+      EdgeLowerBound and the HighestDegree(SpanningTree(Variance)) seed chain
+      (branch_and_bound.cpp:9-115), graph-parallel, on every 4th graph with a
+      budget of 2e6 search nodes per graph."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib
+    from gisbdw_amd import graphs as G
+
+    o = oracle_lib.Oracle()
+    out = {"cores": threads}
+    g = G.workload_dp_single(0)
+
+    def induced(m):
+        mask = (1 << m) - 1
+        return G.Graph(m, [g.adj[v] & mask for v in range(m)], f"{g.name}[:{m}]")
+
+    m = 15
+    t0 = time.perf_counter()
+    o.dyncpu(induced(m))
+    dt = time.perf_counter() - t0
+    while m < g.n and dt * 2.4 < 8.0:
+        m += 1
+        dt *= 2.4
+    t0 = time.perf_counter()
+    o.dyncpu(induced(m))
+    dt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.dp_fast(g, threads=1)
+    flat1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.dp_fast(g, threads=threads)
+    flatn = time.perf_counter() - t0
+    out["dp_n20_single"] = {
+        "faithful_dyncpu_1core": {"vertices": m, "s": dt, "subsets_per_s": (1 << m) / dt},
+        "flat_dp_1core_ms": flat1 * 1e3, f"flat_dp_{threads}cores_ms": flatn * 1e3}
+
+    sample = G.workload_dp_batch(4096, 0)[:256]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda h: o.dp_fast(h, threads=1), sample))
+    dt = time.perf_counter() - t0
+    out["dp_batch4096"] = {"sample_graphs": len(sample), "s": dt, "graphs_per_s": len(sample) / dt}
+
+    sample = G.workload_bnb(256, 0)[::4]
+    budget = 2_000_000
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        res = list(ex.map(lambda h: o.bnb(h, 0, 1, budget), sample))
+    dt = time.perf_counter() - t0
+    cert = sum(1 for r in res if r[3])
+    out["bnb256"] = {"sample_graphs": len(sample), "node_budget_per_graph": budget, "s": dt,
+                     "certified": cert, "certified_graphs_per_s": cert / dt,
+                     "nodes_per_s": sum(r[4] for r in res) / dt}
+    return out
+
+
 def profile_traffic():
     """HBM bytes per level launch from the committed rocprofv3 --pmc summary
     (profiles/dp_level_kernel_n30_traffic.md, calibrated against a plain copy
@@ -183,8 +302,16 @@ def main():
     if args.batch_wg_per_cu:
         ctx.set_param("dp.batch_wg_per_cu", args.batch_wg_per_cu)
 
+    explicit = args.low_bits or args.tiles_per_lane or args.block or args.batch_block or \
+        args.batch_wg_per_cu
+    tuned_ms = None
     if args.workload in ("dp30", "dp20"):
         graphs = G.workload_dp_n30() if args.workload == "dp30" else [G.workload_dp_single(0)]
+        if not explicit and not args.no_autotune:
+            tuned_ms = ctx.autotune(graphs[0].n)
+            print(f"[bench] autotuned: low_bits={ctx.get_param('dp.low_bits')} "
+                  f"tiles_per_lane={ctx.get_param('dp.tiles_per_lane')} "
+                  f"block={ctx.get_param('dp.block')}", file=sys.stderr, flush=True)
         dp = api.DynamicGPU(ctx)
 
         def step():
@@ -238,6 +365,7 @@ def main():
                 "tiles_per_lane": ctx.get_param("dp.tiles_per_lane"),
                 "block": ctx.get_param("dp.block"),
                 "xcd_remap": ctx.get_param("dp.xcd_remap"),
+                "autotuned": tuned_ms is not None,
             },
             "roofline": {
                 "bound": "hbm",
@@ -260,6 +388,8 @@ def main():
     else:
         graphs = G.workload_dp_batch(4096, 0)
         ns, adj = api.pack_graphs(graphs)  # inputs packed once, outside the timing
+        if not explicit and not args.no_autotune:
+            tuned_ms = api.dp_autotune_batch(ctx, graphs)
 
         def step():
             td, par, st = api.dp_solve_batch_arrays(ctx, ns, adj, args.batch_mode)
@@ -289,7 +419,10 @@ def main():
             "config": {"workload": "dp-batch: 4096 G(n,p), n~U{12..24}, p~U[0.15,0.5], seed 0",
                        "graphs": len(graphs), "mode": args.batch_mode,
                        "batch_block": ctx.get_param("dp.batch_block"),
-                       "batch_wg_per_cu": ctx.get_param("dp.batch_wg_per_cu")},
+                       "batch_wg_per_cu": ctx.get_param("dp.batch_wg_per_cu"),
+                       "low_bits": ctx.get_param("dp.low_bits"),
+                       "tiles_per_lane": ctx.get_param("dp.tiles_per_lane"),
+                       "autotuned": tuned_ms is not None},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
             "extra": {"subsets_per_s": sub / wall},
@@ -298,9 +431,15 @@ def main():
     if not args.no_bnb and args.workload == "dp30":
         print("[bench] branch-and-bound extra", file=sys.stderr, flush=True)
         out["extra"]["bnb"] = bnb_extra(ctx)
+    if args.workload == "dp30":
+        out["extra"]["gpu_configs"] = gpu_configs_extra(ctx)
     if not args.no_cpu:
         g0 = graphs[0]
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline_dp(g0)
+        if args.workload == "dp30":
+            print("[bench] cpu paths of the other configurations", file=sys.stderr, flush=True)
+            out["extra"]["cpu_configs"] = cpu_extra(out["cpu_baseline"]["cores"])
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -97,6 +97,8 @@ def load_library():
         "td_dp_solve_batch": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
                                         P(C.c_uint8), P(C.c_int8), C.c_int32, P(DPStats)]),
         "td_dp_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, P(C.c_double)]),
+        "td_dp_autotune_batch": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
+                                           C.c_char_p, P(C.c_double)]),
         "td_ctx_copy_bench": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, P(C.c_double)]),
     }
     sig.update({
@@ -179,6 +181,14 @@ class Context:
 
     def set_param(self, key: str, value: int):
         self.check(self.lib.td_ctx_set_param(self.h, key.encode(), int(value)))
+
+    def autotune(self, n: int, cache_path=None) -> float:
+        """td_dp_autotune: best level-kernel ms for n; the context keeps the
+        winning configuration."""
+        v = C.c_double()
+        cp = str(cache_path).encode() if cache_path is not None else None
+        self.check(self.lib.td_dp_autotune(self.h, n, cp, C.byref(v)))
+        return v.value
 
     def copy_bench(self, nbytes: int, reps: int = 5) -> float:
         """Achieved GB/s of a plain device copy (read + write)."""
@@ -285,6 +295,17 @@ def dp_solve_batch_arrays(ctx: Context, ns: np.ndarray, adj: np.ndarray, mode: i
                                         _ptr(adj, C.c_uint64), _ptr(td, C.c_uint8),
                                         _ptr(par, C.c_int8), mode, C.byref(st)))
     return td, par, st
+
+
+def dp_autotune_batch(ctx: Context, graphs: Sequence[Graph], cache_path=None) -> float:
+    """td_dp_autotune_batch: tunes the persistent batch launch on `graphs`;
+    returns the best level-kernel ms, the context keeps the configuration."""
+    ns, adj = pack_graphs(graphs)
+    v = C.c_double()
+    cp = str(cache_path).encode() if cache_path is not None else None
+    ctx.check(ctx.lib.td_dp_autotune_batch(ctx.h, len(ns), _ptr(ns, C.c_int32),
+                                           _ptr(adj, C.c_uint64), cp, C.byref(v)))
+    return v.value
 
 
 def dp_solve_batch(ctx: Context, graphs: Sequence[Graph], mode: int = 0):

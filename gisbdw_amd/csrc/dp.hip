@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <string>
 #include <vector>
 
 #include "ctx.hpp"
@@ -842,6 +843,110 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
   }
 }
 
+// ---------------------------------------------------------------------------
+// autotuner: a sweep over candidate configurations, timed by the level
+// kernels' HIP events, with results kept as JSON lines in a cache file
+// (one line per candidate, then one "best" line per key and device arch)
+// ---------------------------------------------------------------------------
+bool JsonInt(const std::string& line, const char* name, int* v) {
+  const std::string k = std::string("\"") + name + "\":";
+  const size_t p = line.find(k);
+  return p != std::string::npos && std::sscanf(line.c_str() + p + k.size(), "%d", v) == 1;
+}
+
+bool JsonDouble(const std::string& line, const char* name, double* v) {
+  const std::string k = std::string("\"") + name + "\":";
+  const size_t p = line.find(k);
+  return p != std::string::npos && std::sscanf(line.c_str() + p + k.size(), "%lf", v) == 1;
+}
+
+std::string JsonLine(const std::string& key, const DPConfig& c, double ms, bool best,
+                     const char* arch) {
+  char line[400];
+  std::snprintf(line, sizeof(line),
+                "{\"key\":\"%s\",\"best\":%s,\"low_bits\":%d,\"tiles_per_lane\":%d,"
+                "\"block\":%d,\"batch_block\":%d,\"batch_wg_per_cu\":%d,"
+                "\"level_ms\":%.4f,\"arch\":\"%s\"}\n",
+                key.c_str(), best ? "true" : "false", c.low_bits, c.tiles_per_lane,
+                c.block, c.batch_block, c.batch_wg_per_cu, ms, arch);
+  return line;
+}
+
+// The last "best" line of the cache for this key and arch, if any.
+bool CacheLookup(const char* path, const std::string& key, const char* arch,
+                 DPConfig* cfg, double* ms) {
+  if (!path) return false;
+  std::ifstream f(path);
+  if (!f) return false;
+  const std::string want_key = "\"key\":\"" + key + "\"";
+  const std::string want_arch = std::string("\"arch\":\"") + arch + "\"";
+  bool found = false;
+  std::string line;
+  while (std::getline(f, line)) {
+    if (line.find(want_key) == std::string::npos ||
+        line.find("\"best\":true") == std::string::npos ||
+        line.find(want_arch) == std::string::npos)
+      continue;
+    DPConfig c = *cfg;
+    double m = 0;
+    if (JsonInt(line, "low_bits", &c.low_bits) && JsonInt(line, "tiles_per_lane", &c.tiles_per_lane) &&
+        JsonInt(line, "block", &c.block) && JsonInt(line, "batch_block", &c.batch_block) &&
+        JsonInt(line, "batch_wg_per_cu", &c.batch_wg_per_cu) && JsonDouble(line, "level_ms", &m) &&
+        c.low_bits >= 2 && c.low_bits <= 5 && c.tiles_per_lane >= 1 && c.tiles_per_lane <= 64 &&
+        c.block >= 64 && c.block <= 1024 && c.block % 64 == 0 && c.batch_block >= 64 &&
+        c.batch_block <= 1024 && c.batch_block % 64 == 0 && c.batch_wg_per_cu >= 1 &&
+        c.batch_wg_per_cu <= 32) {
+      *cfg = c;
+      *ms = m;
+      found = true;
+    }
+  }
+  return found;
+}
+
+void ApplyTuned(td_ctx* ctx, const DPConfig& best) {
+  const int use_graph = ctx->dp.use_graph;
+  ctx->dp = best;
+  ctx->dp.use_graph = use_graph;
+  for (auto& kv : ctx->dp_graphs) (void)hipGraphExecDestroy(kv.second);
+  ctx->dp_graphs.clear();
+  ctx->dp_graph_ptrs.clear();
+}
+
+// Times run() (level-kernel ms) under each candidate: one warm-up, then the
+// mean of three.  Returns the best time; *best receives its configuration.
+template <typename Run>
+double Sweep(td_ctx* ctx, const std::string& key, const std::vector<DPConfig>& cands,
+             const char* cache_path, DPConfig* best, Run run) {
+  const DPConfig saved = ctx->dp;
+  double best_ms = 1e30;
+  std::string log;
+  try {
+    for (const DPConfig& c : cands) {
+      ctx->dp = c;
+      run();
+      double t = 0;
+      for (int rep = 0; rep < 3; ++rep) t += run();
+      t /= 3;
+      log += JsonLine(key, c, t, false, ctx->prop.gcnArchName);
+      if (t < best_ms) {
+        best_ms = t;
+        *best = c;
+      }
+    }
+  } catch (...) {
+    ctx->dp = saved;
+    throw;
+  }
+  ctx->dp = saved;
+  if (cache_path) {
+    std::ofstream f(cache_path, std::ios::app);
+    f << log << JsonLine(key, *best, best_ms, true, ctx->prop.gcnArchName);
+    TD_REQUIRE(f.good(), TD_E_INVALID, "cannot write the autotune cache file");
+  }
+  return best_ms;
+}
+
 }  // namespace
 }  // namespace tdhip
 
@@ -943,74 +1048,91 @@ int td_dp_autotune(td_ctx* ctx, int32_t n, const char* cache_path,
                    double* best_ms_out) {
   return Guard(ctx, [&] {
     TD_REQUIRE(n >= 8 && n <= kMaxDPN, TD_E_INVALID, "autotune n in [8, 32]");
-    // seeded G(n, 0.25) (SplitMix64)
-    std::vector<uint64_t> adj(n, 0);
-    uint64_t s = 0x9E3779B97F4A7C15ull * (n + 1);
-    auto next = [&]() {
-      s += 0x9E3779B97F4A7C15ull;
-      uint64_t z = s;
-      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-      return z ^ (z >> 31);
-    };
-    for (int i = 0; i < n; ++i)
-      for (int j = i + 1; j < n; ++j)
-        if ((next() >> 11) * (1.0 / (1ull << 53)) < 0.25) {
-          adj[i] |= 1ull << j;
-          adj[j] |= 1ull << i;
-        }
-    const DPConfig saved = ctx->dp;
-    DPConfig best = saved;
-    double best_ms = 1e30;
-    std::string log;
-    const int lbs[] = {3, 4, 5};
-    const int rs[] = {1, 2, 4, 8};
-    const int blocks[] = {128, 256, 512};
-    for (int lb : lbs)
-      for (int r : rs)
-        for (int b : blocks) {
-          ctx->dp = saved;
-          ctx->dp.low_bits = lb;
-          ctx->dp.tiles_per_lane = r;
-          ctx->dp.block = b;
-          ctx->dp.use_graph = 0;
-          td_dp_stats st{};
-          uint8_t td;
-          int8_t par[64];
-          SolveOne(ctx, n, adj.data(), &td, par, &st);  // warm-up
-          double t = 0;
-          for (int rep = 0; rep < 3; ++rep) {
-            SolveOne(ctx, n, adj.data(), &td, par, &st);
-            t += st.level_ms;
+    const std::string key = "single n=" + std::to_string(n);
+    DPConfig best = ctx->dp;
+    double best_ms = 0;
+    if (!CacheLookup(cache_path, key, ctx->prop.gcnArchName, &best, &best_ms)) {
+      // seeded G(n, 0.25) (SplitMix64)
+      std::vector<uint64_t> adj(n, 0);
+      uint64_t s = 0x9E3779B97F4A7C15ull * (n + 1);
+      auto next = [&]() {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+      };
+      for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j)
+          if ((next() >> 11) * (1.0 / (1ull << 53)) < 0.25) {
+            adj[i] |= 1ull << j;
+            adj[j] |= 1ull << i;
           }
-          t /= 3;
-          char line[160];
-          std::snprintf(line, sizeof(line),
-                        "{\"n\":%d,\"low_bits\":%d,\"tiles_per_lane\":%d,"
-                        "\"block\":%d,\"level_ms\":%.4f}\n",
-                        n, lb, r, b, t);
-          log += line;
-          if (t < best_ms) {
-            best_ms = t;
-            best = ctx->dp;
+      std::vector<DPConfig> cands;
+      for (int lb : {3, 4, 5})
+        for (int r : {1, 2, 4, 8})
+          for (int blk : {128, 256, 512}) {
+            DPConfig c = ctx->dp;
+            c.low_bits = lb;
+            c.tiles_per_lane = r;
+            c.block = blk;
+            c.use_graph = 0;
+            cands.push_back(c);
           }
-        }
-    best.use_graph = saved.use_graph;
-    ctx->dp = best;
-    for (auto& kv : ctx->dp_graphs) (void)hipGraphExecDestroy(kv.second);
-    ctx->dp_graphs.clear();
-    if (cache_path) {
-      std::ofstream f(cache_path, std::ios::app);
-      f << log;
-      char line[200];
-      std::snprintf(line, sizeof(line),
-                    "{\"n\":%d,\"best\":true,\"low_bits\":%d,"
-                    "\"tiles_per_lane\":%d,\"block\":%d,\"level_ms\":%.4f,"
-                    "\"arch\":\"%s\"}\n",
-                    n, best.low_bits, best.tiles_per_lane, best.block, best_ms,
-                    ctx->prop.gcnArchName);
-      f << line;
+      uint8_t td;
+      int8_t par[64];
+      best_ms = Sweep(ctx, key, cands, cache_path, &best, [&] {
+        td_dp_stats st{};
+        SolveOne(ctx, n, adj.data(), &td, par, &st);
+        return st.level_ms;
+      });
     }
+    ApplyTuned(ctx, best);
+    if (best_ms_out) *best_ms_out = best_ms;
+  });
+}
+
+int td_dp_autotune_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
+                         const uint64_t* adj_concat, const char* cache_path,
+                         double* best_ms_out) {
+  return Guard(ctx, [&] {
+    TD_REQUIRE(ngraphs >= 1 && n && adj_concat, TD_E_INVALID,
+               "empty batch or null pointer");
+    int nmax = 0;
+    uint64_t nsum = 0;
+    for (int i = 0; i < ngraphs; ++i) {
+      TD_REQUIRE(n[i] >= 1 && n[i] <= kMaxBatchN, TD_E_INVALID,
+                 "batch graphs need n in [1, 26]");
+      nmax = std::max(nmax, static_cast<int>(n[i]));
+      nsum += static_cast<uint64_t>(n[i]);
+    }
+    const std::string key = "batch graphs=" + std::to_string(ngraphs) +
+                            " nmax=" + std::to_string(nmax) +
+                            " nsum=" + std::to_string(nsum);
+    DPConfig best = ctx->dp;
+    double best_ms = 0;
+    if (!CacheLookup(cache_path, key, ctx->prop.gcnArchName, &best, &best_ms)) {
+      std::vector<DPConfig> cands;
+      for (int lb : {4, 5})
+        for (int r : {1, 2, 4})
+          for (int blk : {256, 512, 1024})
+            for (int wg : {1, 2, 4, 8}) {
+              if (blk * wg > 4096) continue;  // at most 64 waves per CU in flight
+              DPConfig c = ctx->dp;
+              c.low_bits = lb;
+              c.tiles_per_lane = r;
+              c.batch_block = blk;
+              c.batch_wg_per_cu = wg;
+              c.use_graph = 0;
+              cands.push_back(c);
+            }
+      best_ms = Sweep(ctx, key, cands, cache_path, &best, [&] {
+        td_dp_stats st{};
+        SolveBatch(ctx, ngraphs, n, adj_concat, nullptr, nullptr, 0, &st);
+        return st.level_ms;
+      });
+    }
+    ApplyTuned(ctx, best);
     if (best_ms_out) *best_ms_out = best_ms;
   });
 }

@@ -110,10 +110,21 @@ int td_dp_solve_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
 
 /* Autotuner: times the DP level kernel for every candidate (low_bits,
  * tiles_per_lane, block) on a seeded G(n, 0.25) graph and keeps the best in
- * the context (and in cache_path as JSON lines when non-null).  best_ms_out
- * nullable. */
+ * the context.  With a non-null cache_path (JSON lines), a "best" line for
+ * this n and device arch is applied without timing anything; otherwise the
+ * sweep runs and every candidate plus the best are appended.  best_ms_out
+ * (nullable) receives the best level-kernel time.  No reference counterpart:
+ * the reference hard-codes 4096 x 128 threads per chunk
+ * (Treedepth/src/dynamic_gpu/dynamic_gpu.cu:123-124). */
 int td_dp_autotune(td_ctx* ctx, int32_t n, const char* cache_path,
                    double* best_ms_out);
+/* Autotuner of the persistent batch launch (td_dp_solve_batch mode 0):
+ * sweeps (low_bits, tiles_per_lane, batch_block, batch_wg_per_cu) on the
+ * given batch itself and keeps the best; the cache key is the batch's
+ * (graph count, largest n, sum of n).  Same cache rules as td_dp_autotune. */
+int td_dp_autotune_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
+                         const uint64_t* adj_concat, const char* cache_path,
+                         double* best_ms_out);
 
 /* ---------------------------------------------------------------------- */
 /* Branch-and-bound                                                        */

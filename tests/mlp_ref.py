@@ -174,3 +174,43 @@ def init_weights(seed):
             fan = sh[-1]
             w[OFF[k]: OFF[k] + int(np.prod(sh))] = rng.uniform(-1, 1, int(np.prod(sh))) * np.sqrt(6 / fan)
     return w
+
+
+def adam_train(w, graphs, steps, lr, b1=0.9, b2=0.999, eps=1e-8, emulate=True):
+    """Full-batch Adam with bias correction (Kingma and Ba, 2015, Algorithm 1)
+    in FP64: one step per pass over `graphs`.  Returns the loss before each
+    step and the final weights."""
+    w = np.asarray(w, np.float64).copy()
+    m = np.zeros(NPARAMS)
+    v = np.zeros(NPARAMS)
+    curve = []
+    for t in range(1, steps + 1):
+        loss, g = batch_loss_grad(w, graphs, emulate)
+        curve.append(loss)
+        m = b1 * m + (1.0 - b1) * g
+        v = b2 * v + (1.0 - b2) * g * g
+        w = w - lr * (m / (1.0 - b1 ** t)) / (np.sqrt(v / (1.0 - b2 ** t)) + eps)
+    return curve, w
+
+
+def optimal_root_set(oracle, g):
+    """{v : td(G \\ v) = td(G) - 1} of a connected graph, from the oracle DP."""
+    from gisbdw_amd import graphs as G
+
+    td = oracle.dp_fast(g)[0]
+    pos = 0
+    for v in range(g.n):
+        keep = [u for u in range(g.n) if u != v]
+        rest = G.Graph(g.n - 1, [sum(1 << i for i, u in enumerate(keep) if g.adj[x] >> u & 1)
+                                 for x in keep])
+        if oracle.dp_fast(rest)[0] == td - 1:
+            pos |= 1 << v
+    return pos
+
+
+def curve_dataset():
+    """The fixed training set of the loss-curve fixture: 24 connected
+    G(n, p), n = 6..13 cycling, p = 0.25 / 0.35 / 0.45 cycling, seeds 0-23."""
+    from gisbdw_amd import graphs as G
+
+    return [G.gnp_connected(6 + i % 8, (0.25, 0.35, 0.45)[i % 3], i) for i in range(24)]

@@ -264,3 +264,64 @@ def test_deterministic(dp):
     dp(g)
     b = [dp.level(k).tobytes() for k in range(23)]
     assert a == b
+
+
+def test_autotune_single_and_cache(ctx, oracle, tmp_path):
+    """td_dp_autotune sweeps, keeps the fastest candidate in the context and
+    the cache; a second call with the same cache applies the cached line
+    without a sweep (the file does not grow); results stay bit-exact."""
+    import json
+
+    keys = ("dp.low_bits", "dp.tiles_per_lane", "dp.block")
+    saved = {k: ctx.get_param(k) for k in keys}
+    cache = tmp_path / "tune.jsonl"
+    try:
+        best = ctx.autotune(16, cache)
+        lines = [json.loads(x) for x in cache.read_text().splitlines()]
+        cands = [x for x in lines if not x["best"]]
+        win = [x for x in lines if x["best"]]
+        assert len(cands) == 36 and len(win) == 1
+        assert best == pytest.approx(min(x["level_ms"] for x in cands), abs=1e-4)
+        assert best == pytest.approx(win[0]["level_ms"], abs=1e-4)
+        got = {k: ctx.get_param(k) for k in keys}
+        assert got == {"dp.low_bits": win[0]["low_bits"], "dp.tiles_per_lane": win[0]["tiles_per_lane"],
+                       "dp.block": win[0]["block"]}
+        size = cache.stat().st_size
+        for k, v in saved.items():
+            ctx.set_param(k, v)
+        assert ctx.autotune(16, cache) == pytest.approx(best, abs=1e-4)
+        assert cache.stat().st_size == size
+        assert {k: ctx.get_param(k) for k in keys} == got
+        dp = api.DynamicGPU(ctx)
+        for g in (G.gnp(16, 0.3, 8), G.gnp(20, 0.25, 9)):
+            dp(g)
+            td, par = oracle.dp_fast(g)
+            assert (dp.treedepth, dp.parent) == (td, par), g.name
+    finally:
+        for k, v in saved.items():
+            ctx.set_param(k, v)
+
+
+def test_autotune_batch(ctx, oracle, tmp_path):
+    """td_dp_autotune_batch on a small uneven batch: the chosen configuration
+    is one of the candidates, and the batch solved with it equals the oracle."""
+    import json
+
+    keys = ("dp.low_bits", "dp.tiles_per_lane", "dp.batch_block", "dp.batch_wg_per_cu")
+    saved = {k: ctx.get_param(k) for k in keys}
+    cache = tmp_path / "tune.jsonl"
+    gs = G.workload_dp_batch(48, 7)
+    try:
+        best = api.dp_autotune_batch(ctx, gs, cache)
+        lines = [json.loads(x) for x in cache.read_text().splitlines()]
+        win = [x for x in lines if x["best"]]
+        assert len(win) == 1 and best > 0
+        assert ctx.get_param("dp.batch_block") == win[0]["batch_block"]
+        assert ctx.get_param("dp.batch_wg_per_cu") == win[0]["batch_wg_per_cu"]
+        assert ctx.get_param("dp.low_bits") == win[0]["low_bits"]
+        tds, pars, st = api.dp_solve_batch(ctx, gs, 0)
+        for g, t, p in zip(gs, tds, pars):
+            assert (t, p) == oracle.dp_fast(g), g.name
+    finally:
+        for k, v in saved.items():
+            ctx.set_param(k, v)

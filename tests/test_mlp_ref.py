@@ -41,3 +41,32 @@ def test_features_small_cases():
     assert np.allclose(x[:4, 3] * 8, [3, 2, 2, 3])
     x = mlp_ref.features(5, G.complete(5).adj_array())
     assert np.allclose(x[:5, 1] * 4, 4)
+
+
+def _curve_fixture():
+    import json
+    from pathlib import Path
+
+    return json.loads((Path(__file__).resolve().parent / "golden" / "mlp_loss_curve.json").read_text())
+
+
+def test_loss_curve_fixture(oracle):
+    """The committed full-batch Adam trajectory equals the FP64 reference run
+    now, on the same training set and the oracle's optimal-root labels; the
+    loss falls.  Also pins Adam itself: after the first step every weight with
+    a gradient above 1e-4 moves by lr against its gradient's sign."""
+    fx = _curve_fixture()
+    gs = mlp_ref.curve_dataset()
+    assert [g.name for g in gs] == [d["name"] for d in fx["graphs"]]
+    assert [[str(a) for a in g.adj] for g in gs] == [d["adj"] for d in fx["graphs"]]
+    targets = [mlp_ref.optimal_root_set(oracle, g) for g in gs]
+    assert [str(t) for t in targets] == fx["targets"]
+    data = [(g.n, np.array(g.adj + [0] * (64 - g.n), np.uint64), t) for g, t in zip(gs, targets)]
+    w0 = mlp_ref.init_weights(fx["init_seed"]).astype(np.float32).astype(np.float64)
+    curve, w = mlp_ref.adam_train(w0, data, fx["steps"], fx["lr"])
+    assert np.allclose(curve, fx["loss_curve"], rtol=1e-12, atol=0)
+    assert curve[-1] < curve[0]
+    _, g = mlp_ref.batch_loss_grad(w0, data)
+    _, w1 = mlp_ref.adam_train(w0, data, 1, fx["lr"])
+    big = np.abs(g) > 1e-4
+    assert np.allclose((w1 - w0)[big], -fx["lr"] * np.sign(g[big]), rtol=1e-3)
