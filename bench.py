@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--tiles-per-lane", type=int, default=None)
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--xcd-remap", type=int, default=None, choices=[0, 1])
+    ap.add_argument("--class-bits", type=int, default=None, choices=[0, 3, 4])
     ap.add_argument("--batch-block", type=int, default=None)
     ap.add_argument("--batch-wg-per-cu", type=int, default=None)
     ap.add_argument("--no-autotune", action="store_true",
@@ -297,13 +298,15 @@ def main():
         ctx.set_param("dp.block", args.block)
     if args.xcd_remap is not None:
         ctx.set_param("dp.xcd_remap", args.xcd_remap)
+    if args.class_bits is not None:
+        ctx.set_param("dp.class_bits", args.class_bits)
     if args.batch_block:
         ctx.set_param("dp.batch_block", args.batch_block)
     if args.batch_wg_per_cu:
         ctx.set_param("dp.batch_wg_per_cu", args.batch_wg_per_cu)
 
     explicit = args.low_bits or args.tiles_per_lane or args.block or args.batch_block or \
-        args.batch_wg_per_cu
+        args.batch_wg_per_cu or args.class_bits is not None
     tuned_ms = None
     if args.workload in ("dp30", "dp20"):
         graphs = G.workload_dp_n30() if args.workload == "dp30" else [G.workload_dp_single(0)]
@@ -365,6 +368,7 @@ def main():
                 "tiles_per_lane": ctx.get_param("dp.tiles_per_lane"),
                 "block": ctx.get_param("dp.block"),
                 "xcd_remap": ctx.get_param("dp.xcd_remap"),
+                "class_bits": ctx.get_param("dp.class_bits"),
                 "autotuned": tuned_ms is not None,
             },
             "roofline": {

@@ -218,6 +218,12 @@ int td_ctx_set_param(td_ctx* c, const char* key, int64_t v) {
     c->dp.use_graph = v ? 1 : 0;
   } else if (k == "dp.xcd_remap") {
     c->dp.xcd_remap = v ? 1 : 0;
+  } else if (k == "dp.class_bits") {
+    if (v != 0 && v != 3 && v != 4) return bad("must be 0, 3 or 4");
+    c->dp.class_bits = (int)v;
+  } else if (k == "dp.class_min_nh") {
+    if (v < 5 || v > 32) return bad("must be in [5, 32]");
+    c->dp.class_min_nh = (int)v;
   } else if (k.rfind("bnb.", 0) == 0 || k.rfind("mlp.", 0) == 0) {
     c->params[k] = v;
   } else {
@@ -239,6 +245,8 @@ int td_ctx_get_param(const td_ctx* c, const char* key, int64_t* v) {
   else if (k == "dp.batch_block") *v = c->dp.batch_block;
   else if (k == "dp.use_graph") *v = c->dp.use_graph;
   else if (k == "dp.xcd_remap") *v = c->dp.xcd_remap;
+  else if (k == "dp.class_bits") *v = c->dp.class_bits;
+  else if (k == "dp.class_min_nh") *v = c->dp.class_min_nh;
   else {
     auto it = c->params.find(k);
     if (it == c->params.end()) return TD_E_INVALID;

@@ -38,13 +38,44 @@ struct DPGraph {
   uint32_t n;        // vertices
   uint32_t nh;       // high vertices, max(n - lb, 0)
   uint32_t lb;       // low bits per tile
-  uint32_t pad0;
+  uint32_t cb;       // class bits of the level layout (0, 3 or 4; see TileIndex)
   uint64_t table;    // byte offset of this graph's tables in the arena
   uint32_t lvl[34];  // tile offset of high level h; lvl[nh+1] = 2^nh
   uint32_t adj[32];  // adjacency rows
   uint32_t nl[32];   // N(L) for every L subset of the low vertices
   uint32_t nbh[512]; // N(H) by 7-bit chunks of H: nbh[128 c + chunk]
 };
+
+// Class-major level layout.  High level h (|H| = h) starts at tile lvl[h].
+// With cb class bits its tiles are grouped by class p = H & (2^cb - 1), in
+// increasing p, and class p holds the C(nh - cb, h - |p|) sets Hr = H >> cb
+// in colex order.  Removing a vertex of Hr stays in class p; removing a
+// class vertex b keeps Hr and its rank and moves to class p \ b.  cb = 0 is
+// plain colex order of H.  C(x, j) is the binomial coefficient functor.
+template <typename BinomFn>
+__host__ __device__ inline uint32_t ClassBase(int h, uint32_t p, int rest, BinomFn C) {
+  uint32_t base = 0;
+  for (uint32_t q = 0; q < p; ++q) {
+    const int k = h - __builtin_popcount(q);
+    if (k >= 0 && k <= rest) base += static_cast<uint32_t>(C(rest, k));
+  }
+  return base;
+}
+
+// Tile offset of high part H inside its level.
+template <typename BinomFn>
+__host__ __device__ inline uint32_t TileIndex(uint32_t H, int nh, int cb, BinomFn C) {
+  const uint32_t p = H & ((1u << cb) - 1u);
+  uint32_t Hr = H >> cb;
+  uint32_t r = ClassBase(__builtin_popcount(H), p, nh - cb, C);
+  int i = 0;
+  while (Hr) {
+    const int x = __builtin_ctz(Hr);
+    Hr &= Hr - 1;
+    r += static_cast<uint32_t>(C(x, ++i));
+  }
+  return r;
+}
 
 // DP launch configuration (autotuned).
 struct DPConfig {
@@ -55,6 +86,8 @@ struct DPConfig {
   int batch_block = 1024;  // persistent batch kernel: threads per workgroup
   int use_graph = 1;       // capture the level sequence into a hipGraph
   int xcd_remap = 0;       // contiguous tile slice per XCD (measured slower)
+  int class_bits = 0;      // single-graph level layout: 0 (plain colex), 3 or 4
+  int class_min_nh = 16;   // class layout only from this many high vertices
 };
 
 struct DeviceBuffer {

@@ -99,13 +99,18 @@ __device__ __forceinline__ void StoreTile(uint8_t* p,
   }
 }
 
-// Resolve one tile: all 2^LB subsets H u L of high part H (|H| = h, colex
-// rank t among high level h).  prev = first tile of high level h-1, cur =
-// first tile of high level h.
-template <int LB>
+// Resolve one tile: all 2^LB subsets H u L of high part H (|H| = h).
+// Plain layout (kClass = false): t = colex rank of H in high level h, prev =
+// first tile of high level h-1, cur = first tile of high level h, Hr = H,
+// k = h.  Class layout (kClass = true): t = colex rank of Hr = H >> cb among
+// the k-subsets of its class, prev / cur = first tile of class p at levels
+// h-1 / h, and x[0..nx) = first tiles of the classes p \ b (b in p) at level
+// h-1, where the neighbour tile H \ b sits at the same rank t.
+template <int LB, bool kClass>
 __device__ __forceinline__ void DPTile(const uint8_t* __restrict__ prev,
                                        uint8_t* __restrict__ cur, uint32_t t,
-                                       uint32_t H, int h,
+                                       uint32_t Hr, int k, uint32_t H, int h,
+                                       const uint8_t* const (&x)[4], int nx,
                                        const uint32_t* s_cb,
                                        const uint32_t* s_nbh,
                                        const uint32_t (&nl)[1 << LB]) {
@@ -122,40 +127,47 @@ __device__ __forceinline__ void DPTile(const uint8_t* __restrict__ prev,
     mxE[w] = mxO[w] = AsU16x2(0u);
     orr[w] = 0u;
   }
-  // neighbour tiles H \ x_j, j = h-1 .. 0 (x_j = j-th smallest of H):
-  // rank(H \ x_j) = t - C(x_j, j+1) - sum_{i>j} (C(x_i, i+1) - C(x_i, i)).
-  // Four neighbour tiles per step: h is uniform across the wave, so the
-  // guards below are uniform and the four loads are in flight together.
-  uint32_t rem = H;
+  // neighbour tiles Hr \ y_j, j = k-1 .. 0 (y_j = j-th smallest of Hr):
+  // rank(Hr \ y_j) = t - C(y_j, j+1) - sum_{i>j} (C(y_i, i+1) - C(y_i, i)),
+  // then (class layout) the nx tiles of the classes p \ b at rank t.
+  // Four neighbour tiles per step: k and nx are uniform across the wave, so
+  // the guards below are uniform and the four loads are in flight together.
+  uint32_t rem = Hr;
   uint32_t A = 0;
-  for (int j0 = h - 1; j0 >= 0; j0 -= kNbUnroll) {
+  const int total = kClass ? k + nx : k;
+  for (int i0 = 0; i0 < total; i0 += kNbUnroll) {
     uint32_t wv[kNbUnroll][NW];
 #pragma unroll
     for (int u = 0; u < kNbUnroll; ++u) {
-      const int j = j0 - u;
-      if (j >= 0) {
-        const int x = 31 - __builtin_clz(rem);
-        rem ^= 1u << x;
-        const uint32_t c1 = s_cb[x * kCbStride + j + 1];
-        const uint32_t c0 = s_cb[x * kCbStride + j];
+      const int i = i0 + u;
+      if (i < k) {
+        const int j = k - 1 - i;
+        const int y = 31 - __builtin_clz(rem);
+        rem ^= 1u << y;
+        const uint32_t c1 = s_cb[y * kCbStride + j + 1];
+        const uint32_t c0 = s_cb[y * kCbStride + j];
         const uint32_t r2 = t - c1 - A;
         A += c1 - c0;
         LoadTile<LB>(prev + (static_cast<size_t>(r2) << LB), wv[u]);
+      } else if (kClass && i < total) {
+        const int c = i - k;
+        const uint8_t* base = c == 0 ? x[0] : c == 1 ? x[1] : c == 2 ? x[2] : x[3];
+        LoadTile<LB>(base + (static_cast<size_t>(t) << LB), wv[u]);
       }
     }
 #pragma unroll
     for (int u = 0; u < kNbUnroll; ++u) {
-      if (j0 - u >= 0) {
+      if (i0 + u < total) {
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-          const uint32_t x = wv[u][w];
-          const u16x2 xe = AsU16x2(__builtin_amdgcn_perm(x, x, 0x0c020c00u));
-          const u16x2 xo = AsU16x2(__builtin_amdgcn_perm(x, x, 0x0c030c01u));
+          const uint32_t xw = wv[u][w];
+          const u16x2 xe = AsU16x2(__builtin_amdgcn_perm(xw, xw, 0x0c020c00u));
+          const u16x2 xo = AsU16x2(__builtin_amdgcn_perm(xw, xw, 0x0c030c01u));
           mnE[w] = __builtin_elementwise_min(mnE[w], xe);
           mnO[w] = __builtin_elementwise_min(mnO[w], xo);
           mxE[w] = __builtin_elementwise_max(mxE[w], xe);
           mxO[w] = __builtin_elementwise_max(mxO[w], xo);
-          orr[w] |= x;
+          orr[w] |= xw;
         }
       }
     }
@@ -227,11 +239,12 @@ __device__ __forceinline__ void DPLevelRange(const DPGraph* __restrict__ g,
   const uint8_t* prev =
       h > 0 ? base + (static_cast<size_t>(g->lvl[h - 1]) << LB) : base;
   uint8_t* cur = base + (static_cast<size_t>(g->lvl[h]) << LB);
+  const uint8_t* const none[4] = {nullptr, nullptr, nullptr, nullptr};
   for (uint32_t t0 = first + lane_id * R; t0 < end; t0 += stride) {
     uint32_t H = Unrank(t0, h, nh, s_cb);
     const uint32_t tend = min(t0 + static_cast<uint32_t>(R), end);
     for (uint32_t t = t0;;) {
-      DPTile<LB>(prev, cur, t, H, h, s_cb, s_nbh, nl);
+      DPTile<LB, false>(prev, cur, t, H, h, H, h, none, 0, s_cb, s_nbh, nl);
       if (++t >= tend) break;
       H = NextCombination(H);
     }
@@ -259,6 +272,71 @@ __global__ __launch_bounds__(1024) void DPLevelKernel(
   const uint32_t lane_id = bid * blockDim.x + threadIdx.x;
   const uint32_t stride = gridDim.x * blockDim.x * R;
   DPLevelRange<LB>(g, arena, h, 0, ntiles, lane_id, stride, R, s_cb, s_nbh);
+}
+
+// Static schedule of one class-layout level launch: for each XCD, up to
+// kMaxSeg segments (class p, tiles [first, end) of that class), with the tile
+// offsets of class p at levels h-1 and h and of the classes p \ b at h-1.
+constexpr int kMaxSeg = 12;
+struct ClassSeg {
+  uint32_t p, first, end, prev, cur, x[4];
+};
+struct ClassSched {
+  uint32_t nseg[kXcds];
+  ClassSeg seg[kXcds][kMaxSeg];
+};
+
+// One high level of one graph in the class layout (cb = 3 or 4).  The host
+// gives each XCD its own classes (class p belongs to XCD p mod 8 for cb = 3;
+// for cb = 4 to XCD p and 15 - p, one class of each complementary pair) and
+// moves the tail of over-full XCDs' classes to the others, so every XCD gets
+// the same number of tiles.  Workgroups are dealt to XCDs round-robin, so
+// workgroup w works for XCD w mod 8: the XCD's workgroups walk its segments
+// in order, chunk c of a segment going to its workgroup c mod (grid / 8).
+// An XCD's neighbour reads stay in its classes' part of level h-1 and in its
+// own L2.  No atomics and no barriers.
+template <int LB>
+__global__ __launch_bounds__(1024) void DPClassLevelKernel(
+    const DPGraph* __restrict__ g, const uint32_t* __restrict__ g_cb,
+    uint8_t* __restrict__ arena, int h, int R, const ClassSched sched) {
+  __shared__ uint32_t s_cb[32 * kCbStride];
+  __shared__ uint32_t s_nbh[512];
+  LoadShared<LB>(g, g_cb, s_cb, s_nbh, true);
+  __syncthreads();
+  const int cb = static_cast<int>(g->cb);
+  const int rest = static_cast<int>(g->nh) - cb;
+  const uint32_t CH = blockDim.x * static_cast<uint32_t>(R);
+  const uint32_t xcd = blockIdx.x % kXcds;
+  const uint32_t wx = blockIdx.x / kXcds, gx = gridDim.x / kXcds;
+  uint32_t nl[1 << LB];
+#pragma unroll
+  for (int L = 0; L < (1 << LB); ++L) nl[L] = g->nl[L];
+  uint8_t* base = arena + g->table;
+  const uint32_t nseg = sched.nseg[xcd];
+  for (uint32_t s = 0; s < nseg; ++s) {
+    const ClassSeg& sg = sched.seg[xcd][s];
+    const uint32_t p = sg.p;
+    const int np = __builtin_popcount(p);
+    const int k = h - np;
+    const uint8_t* prev = base + (static_cast<size_t>(sg.prev) << LB);
+    uint8_t* cur = base + (static_cast<size_t>(sg.cur) << LB);
+    const uint8_t* const x[4] = {base + (static_cast<size_t>(sg.x[0]) << LB),
+                                 base + (static_cast<size_t>(sg.x[1]) << LB),
+                                 base + (static_cast<size_t>(sg.x[2]) << LB),
+                                 base + (static_cast<size_t>(sg.x[3]) << LB)};
+    for (uint32_t c0 = sg.first + wx * CH; c0 < sg.end; c0 += gx * CH) {
+      const uint32_t t0 = c0 + threadIdx.x * static_cast<uint32_t>(R);
+      if (t0 >= sg.end) continue;
+      uint32_t Hr = Unrank(t0, k, rest, s_cb);
+      const uint32_t tend = min(t0 + static_cast<uint32_t>(R), sg.end);
+      for (uint32_t t = t0;;) {
+        const uint32_t H = p | (Hr << cb);
+        DPTile<LB, true>(prev, cur, t, Hr, k, H, h, x, np, s_cb, s_nbh, nl);
+        if (++t >= tend) break;
+        Hr = NextCombination(Hr);
+      }
+    }
+  }
 }
 
 // Level-synchronous batch: one launch per high level h over all graphs;
@@ -324,15 +402,10 @@ __device__ __forceinline__ uint32_t Lookup(const DPGraph* __restrict__ g,
                                            const uint32_t* s_cb, uint32_t S) {
   const uint32_t lb = g->lb;
   const uint32_t L = S & ((1u << lb) - 1u);
-  uint32_t H = S >> lb;
+  const uint32_t H = S >> lb;
   const int h = __builtin_popcount(H);
-  uint32_t r = 0;
-  int i = 0;
-  while (H) {
-    const int x = __builtin_ctz(H);
-    H &= H - 1;
-    r += s_cb[x * kCbStride + (++i)];
-  }
+  const uint32_t r = TileIndex(H, static_cast<int>(g->nh), static_cast<int>(g->cb),
+                               [&](int x, int j) { return s_cb[x * kCbStride + j]; });
   const size_t addr =
       g->table + ((static_cast<size_t>(g->lvl[h]) + r) << lb) + L;
   return static_cast<uint32_t>(arena[addr]) >> 1;
@@ -419,12 +492,13 @@ constexpr int kMaxBatchN = 26;
 
 // Tile offsets of each high level, each level padded to a 256-byte boundary
 // so no cache line holds tiles of two levels.
-DPGraph MakeGraph(int n, const uint64_t* adj, int lb, uint64_t table_off) {
+DPGraph MakeGraph(int n, const uint64_t* adj, int lb, uint64_t table_off, int cb = 0) {
   DPGraph G;
   std::memset(&G, 0, sizeof(G));
   G.n = n;
   G.lb = lb;
   G.nh = n > lb ? n - lb : 0;
+  G.cb = static_cast<uint32_t>(cb);
   G.table = table_off;
   const uint32_t pad_tiles = std::max(1u, 256u >> lb);
   uint64_t off = 0;
@@ -478,10 +552,84 @@ int GridFor(td_ctx* ctx, uint64_t lanes_needed, int block) {
   return static_cast<int>(std::max<uint64_t>(1, std::min(blocks, cap)));
 }
 
+// Balanced static schedule of level h (see DPClassLevelKernel).
+ClassSched MakeClassSched(const DPGraph& G, int h) {
+  const int cb = static_cast<int>(G.cb);
+  const int P = 1 << cb;
+  const int rest = static_cast<int>(G.nh) - cb;
+  auto C = [](int x, int j) { return Binom(x, j); };
+  std::vector<uint64_t> size(P), taken(P, 0);
+  uint64_t total = 0;
+  for (int q = 0; q < P; ++q) {
+    const int k = h - __builtin_popcount(static_cast<unsigned>(q));
+    size[q] = (k < 0 || k > rest) ? 0 : Binom(rest, k);
+    total += size[q];
+  }
+  const uint64_t target = (total + kXcds - 1) / kXcds;
+  ClassSched S;
+  std::memset(&S, 0, sizeof(S));
+  std::vector<uint64_t> load(kXcds, 0);
+  auto add = [&](int x, int q, uint64_t first, uint64_t end) {
+    TD_REQUIRE(S.nseg[x] < static_cast<uint32_t>(kMaxSeg), TD_E_INVALID,
+               "class schedule needs too many segments");
+    ClassSeg& s = S.seg[x][S.nseg[x]++];
+    s.p = static_cast<uint32_t>(q);
+    s.first = static_cast<uint32_t>(first);
+    s.end = static_cast<uint32_t>(end);
+    s.prev = h > 0 ? G.lvl[h - 1] + ClassBase(h - 1, s.p, rest, C) : 0u;
+    s.cur = G.lvl[h] + ClassBase(h, s.p, rest, C);
+    uint32_t m = s.p;
+    for (int c = 0; c < 4; ++c) {
+      s.x[c] = 0;
+      if (m) {
+        const uint32_t b = m & (0u - m);
+        m ^= b;
+        s.x[c] = G.lvl[h - 1] + ClassBase(h - 1, s.p ^ b, rest, C);
+      }
+    }
+    load[x] += end - first;
+  };
+  // own classes first, from the start of each class, up to the target
+  for (int q = 0; q < P; ++q) {
+    const int x = (P == 16 && q >= 8) ? 15 - q : q % static_cast<int>(kXcds);
+    const uint64_t take = std::min(size[q] - taken[q], target - std::min(target, load[x]));
+    if (take > 0) {
+      add(x, q, taken[q], taken[q] + take);
+      taken[q] += take;
+    }
+  }
+  // the rest of every class to the XCDs below the target
+  for (int q = 0; q < P; ++q) {
+    while (taken[q] < size[q]) {
+      int x = 0;
+      for (int y = 1; y < static_cast<int>(kXcds); ++y)
+        if (load[y] < load[x]) x = y;
+      const uint64_t room = load[x] < target ? target - load[x] : size[q] - taken[q];
+      const uint64_t take = std::min(size[q] - taken[q], room);
+      add(x, q, taken[q], taken[q] + take);
+      taken[q] += take;
+    }
+  }
+  return S;
+}
+
 template <int LB>
 void LaunchLevels(td_ctx* ctx, const DPGraph& G, const DPGraph* dG,
                   const uint32_t* cb, uint8_t* arena, uint64_t* launches) {
   const int R = ctx->dp.tiles_per_lane, block = ctx->dp.block;
+  if (G.cb > 0) {
+    for (int h = 0; h <= static_cast<int>(G.nh); ++h) {
+      const ClassSched sched = MakeClassSched(G, h);
+      const uint64_t ntiles = Binom(G.nh, h);
+      int grid = GridFor(ctx, (ntiles + R - 1) / R, block);
+      grid = std::max(static_cast<int>(kXcds), (grid + 7) / 8 * 8);
+      hipLaunchKernelGGL(DPClassLevelKernel<LB>, dim3(grid), dim3(block), 0,
+                         ctx->stream, dG, cb, arena, h, R, sched);
+      ++*launches;
+    }
+    TD_HIP(hipGetLastError());
+    return;
+  }
   for (int h = 0; h <= static_cast<int>(G.nh); ++h) {
     const uint32_t ntiles = static_cast<uint32_t>(Binom(G.nh, h));
     int grid = GridFor(ctx, (ntiles + R - 1) / R, block);
@@ -513,8 +661,8 @@ void RunLevels(td_ctx* ctx, const DPGraph& G, const DPGraph* dG,
     return;
   }
   // key includes the buffer addresses baked into the captured launches
-  td_ctx::GraphKey key{static_cast<int>(G.n), static_cast<int>(G.lb),
-                       ctx->dp.tiles_per_lane, ctx->dp.block};
+  td_ctx::GraphKey key{static_cast<int>(G.n) | static_cast<int>(G.cb << 8),
+                       static_cast<int>(G.lb), ctx->dp.tiles_per_lane, ctx->dp.block};
   auto& p = ctx->dp_graph_ptrs[key];
   auto it = ctx->dp_graphs.find(key);
   if (it != ctx->dp_graphs.end() &&
@@ -615,7 +763,12 @@ void SolveOne(td_ctx* ctx, int n, const uint64_t* adj, uint8_t* td_out,
   ValidateGraph(n, adj, kMaxDPN);
   // the N(H) chunk tables cover 28 high vertices
   const int lb = std::max(ctx->dp.low_bits, n - 28);
-  DPGraph G = MakeGraph(n, adj, lb, 0);
+  const int nh = n > lb ? n - lb : 0;
+  const int cbits = (ctx->dp.class_bits > 0 && nh >= ctx->dp.class_min_nh &&
+                     nh > ctx->dp.class_bits)
+                        ? ctx->dp.class_bits
+                        : 0;
+  DPGraph G = MakeGraph(n, adj, lb, 0, cbits);
   const uint64_t tbytes = TableBytes(G);
   uint8_t* arena = static_cast<uint8_t*>(ctx->Buffer("dp.arena", tbytes));
   DPGraph* dG = static_cast<DPGraph*>(ctx->Buffer("dp.graph", sizeof(DPGraph)));
@@ -865,10 +1018,10 @@ std::string JsonLine(const std::string& key, const DPConfig& c, double ms, bool 
   char line[400];
   std::snprintf(line, sizeof(line),
                 "{\"key\":\"%s\",\"best\":%s,\"low_bits\":%d,\"tiles_per_lane\":%d,"
-                "\"block\":%d,\"batch_block\":%d,\"batch_wg_per_cu\":%d,"
-                "\"level_ms\":%.4f,\"arch\":\"%s\"}\n",
+                "\"block\":%d,\"class_bits\":%d,\"batch_block\":%d,"
+                "\"batch_wg_per_cu\":%d,\"level_ms\":%.4f,\"arch\":\"%s\"}\n",
                 key.c_str(), best ? "true" : "false", c.low_bits, c.tiles_per_lane,
-                c.block, c.batch_block, c.batch_wg_per_cu, ms, arch);
+                c.block, c.class_bits, c.batch_block, c.batch_wg_per_cu, ms, arch);
   return line;
 }
 
@@ -892,6 +1045,8 @@ bool CacheLookup(const char* path, const std::string& key, const char* arch,
     if (JsonInt(line, "low_bits", &c.low_bits) && JsonInt(line, "tiles_per_lane", &c.tiles_per_lane) &&
         JsonInt(line, "block", &c.block) && JsonInt(line, "batch_block", &c.batch_block) &&
         JsonInt(line, "batch_wg_per_cu", &c.batch_wg_per_cu) && JsonDouble(line, "level_ms", &m) &&
+        JsonInt(line, "class_bits", &c.class_bits) &&
+        (c.class_bits == 0 || c.class_bits == 3 || c.class_bits == 4) &&
         c.low_bits >= 2 && c.low_bits <= 5 && c.tiles_per_lane >= 1 && c.tiles_per_lane <= 64 &&
         c.block >= 64 && c.block <= 1024 && c.block % 64 == 0 && c.batch_block >= 64 &&
         c.batch_block <= 1024 && c.batch_block % 64 == 0 && c.batch_wg_per_cu >= 1 &&
@@ -975,8 +1130,10 @@ int td_dp_get_treedepth(td_ctx* ctx, int32_t k, uint64_t code,
     const uint64_t L = S & ((1ull << G.lb) - 1);
     const uint64_t H = S >> G.lb;
     const int h = __builtin_popcountll(H);
-    const uint64_t addr =
-        ((static_cast<uint64_t>(G.lvl[h]) + RankColex(H)) << G.lb) + L;
+    const uint64_t r = TileIndex(static_cast<uint32_t>(H), static_cast<int>(G.nh),
+                                 static_cast<int>(G.cb),
+                                 [](int x, int j) { return Binom(x, j); });
+    const uint64_t addr = ((static_cast<uint64_t>(G.lvl[h]) + r) << G.lb) + L;
     uint8_t raw = 0;
     const uint8_t* arena = static_cast<const uint8_t*>(ctx->bufs["dp.arena"].p);
     TD_HIP(hipMemcpy(&raw, arena + addr, 1, hipMemcpyDeviceToHost));
@@ -1069,16 +1226,18 @@ int td_dp_autotune(td_ctx* ctx, int32_t n, const char* cache_path,
             adj[j] |= 1ull << i;
           }
       std::vector<DPConfig> cands;
-      for (int lb : {3, 4, 5})
-        for (int r : {1, 2, 4, 8})
-          for (int blk : {128, 256, 512}) {
-            DPConfig c = ctx->dp;
-            c.low_bits = lb;
-            c.tiles_per_lane = r;
-            c.block = blk;
-            c.use_graph = 0;
-            cands.push_back(c);
-          }
+      for (int cbits : {0, 4})
+        for (int lb : {3, 4, 5})
+          for (int r : {1, 2, 4, 8})
+            for (int blk : {128, 256, 512}) {
+              DPConfig c = ctx->dp;
+              c.class_bits = cbits;
+              c.low_bits = lb;
+              c.tiles_per_lane = r;
+              c.block = blk;
+              c.use_graph = 0;
+              cands.push_back(c);
+            }
       uint8_t td;
       int8_t par[64];
       best_ms = Sweep(ctx, key, cands, cache_path, &best, [&] {

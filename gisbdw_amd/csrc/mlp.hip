@@ -455,21 +455,16 @@ __global__ void MLPToBf16Kernel(const float* __restrict__ w, uint16_t* __restric
 // set S (one per wave), positives = {v in S : td(S \ v) = td(S) - 1}.
 struct DPLookup {
   const uint8_t* arena;
-  uint32_t lb;
+  uint32_t lb, nh, cb;
   uint32_t lvl[34];
 };
 
 __device__ __forceinline__ uint32_t LookupTd(const DPLookup& L, const uint32_t* cb, uint32_t S) {
   const uint32_t low = S & ((1u << L.lb) - 1u);
-  uint32_t H = S >> L.lb;
+  const uint32_t H = S >> L.lb;
   const int h = __builtin_popcount(H);
-  uint32_t r = 0;
-  int i = 0;
-  while (H) {
-    const int x = __builtin_ctz(H);
-    H &= H - 1;
-    r += cb[x * 33 + (++i)];
-  }
+  const uint32_t r = TileIndex(H, static_cast<int>(L.nh), static_cast<int>(L.cb),
+                               [&](int x, int j) { return cb[x * 33 + j]; });
   return static_cast<uint32_t>(L.arena[((static_cast<size_t>(L.lvl[h]) + r) << L.lb) + low]) >> 1;
 }
 
@@ -804,6 +799,8 @@ int td_mlp_make_samples(td_ctx* ctx, int32_t count, uint64_t seed, int32_t* n_ou
     DPLookup L;
     L.arena = static_cast<const uint8_t*>(ctx->bufs["dp.arena"].p);
     L.lb = G.lb;
+    L.nh = G.nh;
+    L.cb = G.cb;
     std::memcpy(L.lvl, G.lvl, sizeof(L.lvl));
     const uint32_t* cb = static_cast<const uint32_t*>(ctx->bufs["dp.pascal"].p);
     hipLaunchKernelGGL(LabelKernel, dim3(count), dim3(64), 0, ctx->stream, L, cb, dsets, dpos,

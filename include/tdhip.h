@@ -42,7 +42,10 @@ void td_ctx_destroy(td_ctx* ctx);
 const char* td_last_error(const td_ctx* ctx);
 /* Tuning knobs (see td_dp_autotune): "dp.low_bits" (2..5, default 5),
  * "dp.tiles_per_lane" (1..64, default 1), "dp.block" (64..1024, default 256),
- * "dp.batch_wg_per_cu", "dp.use_graph" (0/1), "bnb.waves", "bnb.node_budget"
+ * "dp.batch_wg_per_cu", "dp.use_graph" (0/1), "dp.class_bits" (0, 3 or 4,
+ * default 0: level tables grouped into 2^bits vertex classes, one class per
+ * XCD, for single graphs with at least "dp.class_min_nh" (default 16)
+ * vertices above the low bits), "bnb.waves", "bnb.node_budget"
  * (per graph, default 2e6), "bnb.waves_per_graph" (1..4, default 4),
  * "bnb.groups" (workgroups in flight), "bnb.memo_log2" (default 17). */
 int td_ctx_set_param(td_ctx* ctx, const char* key, int64_t value);
@@ -108,8 +111,8 @@ int td_dp_solve_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
                       int8_t* parent_concat, int32_t mode,
                       td_dp_stats* stats);
 
-/* Autotuner: times the DP level kernel for every candidate (low_bits,
- * tiles_per_lane, block) on a seeded G(n, 0.25) graph and keeps the best in
+/* Autotuner: times the DP level kernel for every candidate (class_bits,
+ * low_bits, tiles_per_lane, block) on a seeded G(n, 0.25) graph and keeps the best in
  * the context.  With a non-null cache_path (JSON lines), a "best" line for
  * this n and device arch is applied without timing anything; otherwise the
  * sweep runs and every candidate plus the best are appended.  best_ms_out

@@ -153,16 +153,57 @@ def test_config1_graph(dp, oracle):
     assert st.alg_bytes == (1 << 19) * 22
 
 
-@pytest.mark.parametrize("graph", ["gnp24", "grid"])
-def test_n24_vs_oracle(dp, oracle, graph):
+@pytest.mark.parametrize("graph,class_bits", [("gnp24", 0), ("gnp24", 4), ("grid", 4)])
+def test_n24_vs_oracle(ctx, dp, oracle, graph, class_bits):
     g = G.gnp(24, 0.25, 11) if graph == "gnp24" else G.grid_dropped(4, 6, 0.1, 3)
-    print("n24", graph, flush=True)
+    print("n24", graph, class_bits, flush=True)
     td, par, lv = oracle.dp_fast(g, levels=True)
-    dp(g)
-    assert (dp.treedepth, dp.parent) == (td, par)
-    off = level_offsets(g.n)
-    for k in (1, 5, 12, 23, 24):
-        assert np.array_equal(dp.level(k), lv[off[k]: off[k + 1]])
+    ctx.set_param("dp.class_bits", class_bits)
+    try:
+        dp(g)
+        assert (dp.treedepth, dp.parent) == (td, par)
+        off = level_offsets(g.n)
+        for k in (1, 5, 12, 23, 24):
+            assert np.array_equal(dp.level(k), lv[off[k]: off[k + 1]])
+    finally:
+        ctx.set_param("dp.class_bits", 0)
+
+
+@pytest.mark.parametrize("class_bits,low_bits,tiles", [(3, 5, 1), (4, 5, 1), (4, 2, 3),
+                                                       (4, 4, 4), (3, 3, 2)])
+def test_class_layout_all_levels_bit_exact(ctx, oracle, class_bits, low_bits, tiles):
+    """The class-major level layout (vertex classes per XCD, work stealing
+    between classes) at small n: every level equal to the oracle's, the
+    canonical tree, and GetElimination from sampled subsets."""
+    ctx.set_param("dp.class_bits", class_bits)
+    ctx.set_param("dp.class_min_nh", 5)
+    ctx.set_param("dp.low_bits", low_bits)
+    ctx.set_param("dp.tiles_per_lane", tiles)
+    try:
+        dp = api.DynamicGPU(ctx)
+        rng = random.Random(77 + class_bits * 10 + low_bits)
+        for trial in range(8):
+            n = rng.randint(low_bits + class_bits + 1, 20)
+            p = rng.choice([0.1, 0.2, 0.35, 0.6])
+            g = G.gnp(n, p, rng.getrandbits(32))
+            print("class levels", class_bits, low_bits, tiles, n, p, flush=True)
+            td, par, lv = oracle.dp_fast(g, levels=True)
+            dp(g)
+            assert dp.treedepth == td and dp.parent == par
+            off = level_offsets(n)
+            for k in range(n + 1):
+                assert np.array_equal(dp.level(k), lv[off[k]: off[k + 1]]), (n, k)
+            for _ in range(4):
+                S = rng.getrandbits(n) or 1
+                k = bin(S).count("1")
+                got = dp.GetElimination(n, k, colex_rank(S))
+                assert got == ref_elimination(g, lv, S, k)
+                assert dp.GetTreedepth(n, k, colex_rank(S)) == lv[off[k] + colex_rank(S)]
+    finally:
+        ctx.set_param("dp.class_bits", 0)
+        ctx.set_param("dp.class_min_nh", 16)
+        ctx.set_param("dp.low_bits", 5)
+        ctx.set_param("dp.tiles_per_lane", 1)
 
 
 def check_level_recurrence(g, prev, cur, k, samples, rng):
@@ -187,15 +228,19 @@ def check_level_recurrence(g, prev, cur, k, samples, rng):
         assert int(cur[r]) == want, (k, r)
 
 
-@pytest.mark.parametrize("which,remap", [(0, 1), (4, 1), (1, 0)])
-def test_n30_properties(ctx, dp, oracle, which, remap):
+@pytest.mark.parametrize("which,class_bits,remap", [(0, 4, 0), (4, 3, 0), (1, 0, 1), (5, 0, 0)])
+def test_n30_properties(ctx, dp, oracle, which, class_bits, remap):
     """Full-size: certified tree, depth = td, and the recurrence holds on
     sampled subsets of the middle levels read back from the device."""
     g = G.workload_dp_n30()[which]
-    print("n30", g.name, remap, flush=True)
+    print("n30", g.name, class_bits, remap, flush=True)
     ctx.set_param("dp.xcd_remap", remap)
-    dp(g)
-    ctx.set_param("dp.xcd_remap", 0)
+    ctx.set_param("dp.class_bits", class_bits)
+    try:
+        dp(g)
+    finally:
+        ctx.set_param("dp.xcd_remap", 0)
+        ctx.set_param("dp.class_bits", 0)
     assert oracle.certify(g, [int(p) for p in dp.parent], dp.treedepth)
     rng = random.Random(99)
     for k in (2, 14, 15, 29, 30):
@@ -272,7 +317,7 @@ def test_autotune_single_and_cache(ctx, oracle, tmp_path):
     without a sweep (the file does not grow); results stay bit-exact."""
     import json
 
-    keys = ("dp.low_bits", "dp.tiles_per_lane", "dp.block")
+    keys = ("dp.low_bits", "dp.tiles_per_lane", "dp.block", "dp.class_bits")
     saved = {k: ctx.get_param(k) for k in keys}
     cache = tmp_path / "tune.jsonl"
     try:
@@ -280,12 +325,12 @@ def test_autotune_single_and_cache(ctx, oracle, tmp_path):
         lines = [json.loads(x) for x in cache.read_text().splitlines()]
         cands = [x for x in lines if not x["best"]]
         win = [x for x in lines if x["best"]]
-        assert len(cands) == 36 and len(win) == 1
+        assert len(cands) == 72 and len(win) == 1
         assert best == pytest.approx(min(x["level_ms"] for x in cands), abs=1e-4)
         assert best == pytest.approx(win[0]["level_ms"], abs=1e-4)
         got = {k: ctx.get_param(k) for k in keys}
         assert got == {"dp.low_bits": win[0]["low_bits"], "dp.tiles_per_lane": win[0]["tiles_per_lane"],
-                       "dp.block": win[0]["block"]}
+                       "dp.block": win[0]["block"], "dp.class_bits": win[0]["class_bits"]}
         size = cache.stat().st_size
         for k, v in saved.items():
             ctx.set_param(k, v)

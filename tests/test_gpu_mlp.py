@@ -75,10 +75,17 @@ def test_scores_and_grad_match_reference(ctx):
         assert rel < 2e-2 and cos > 0.999, (k, rel, cos)
 
 
-def test_labels_are_optimal_roots(ctx, oracle):
+@pytest.mark.parametrize("class_bits", [0, 4])
+def test_labels_are_optimal_roots(ctx, oracle, class_bits):
     dp = api.DynamicGPU(ctx)
     g = G.gnp(16, 0.3, 4)
-    dp(g)
+    ctx.set_param("dp.class_bits", class_bits)
+    ctx.set_param("dp.class_min_nh", 5)
+    try:
+        dp(g)
+    finally:
+        ctx.set_param("dp.class_bits", 0)
+        ctx.set_param("dp.class_min_nh", 16)
     import ctypes as C
 
     k = 40
