@@ -370,3 +370,32 @@ def test_autotune_batch(ctx, oracle, tmp_path):
     finally:
         for k, v in saved.items():
             ctx.set_param(k, v)
+
+
+@pytest.mark.parametrize("graph", ["gnp32", "grid4x8", "gnp31"])
+def test_max_size_properties(ctx, dp, oracle, graph):
+    """The largest single graphs (n = 31, 32: 2^32 subsets, 4 GiB of level
+    tables): the returned tree passes the certifier with depth = td, and on
+    sampled subsets of several levels the stored td (td_dp_get_treedepth)
+    satisfies the recurrence against the stored values of S \\ v."""
+    g = {"gnp32": lambda: G.gnp(32, 0.15, 21), "grid4x8": lambda: G.grid(4, 8),
+         "gnp31": lambda: G.gnp(31, 0.2, 4)}[graph]()
+    print("max size", graph, flush=True)
+    dp(g)
+    n = g.n
+    assert oracle.certify(g, [int(p) for p in dp.parent], dp.treedepth)
+    assert dp.GetTreedepth(n, n, 0) == dp.treedepth
+    rng = random.Random(7)
+
+    def td_of(S):
+        k = bin(S).count("1")
+        return dp.GetTreedepth(n, k, colex_rank(S))
+
+    for k in (2, 9, 16, n - 1):
+        for _ in range(40):
+            S = 0
+            while bin(S).count("1") < k:
+                S |= 1 << rng.randrange(n)
+            vals = [td_of(S & ~(1 << v)) for v in range(n) if S >> v & 1]
+            want = 1 + min(vals) if len(components(g.adj, S)) == 1 else max(vals)
+            assert td_of(S) == want, (k, S)
