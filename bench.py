@@ -124,6 +124,22 @@ def bnb_extra(ctx, budget=50000):
         "learned_over_plain": (sum(certl) / wall_l) / max(sum(cert) / wall_p, 1e-9),
         "certified_answers_agree": agree,
     }
+    # the same comparison over per-graph node budgets (scores computed once)
+    sweep = []
+    for b in (1000, 3000, 10000, 30000, 100000):
+        ctx.set_param("bnb.node_budget", b)
+        row = {"node_budget_per_graph": b}
+        for name, sc in (("plain", None), ("learned", scores)):
+            t0 = time.perf_counter()
+            _, _, c, s = api.bnb_solve_batch(ctx, W, scores=sc)
+            wall = time.perf_counter() - t0
+            row[name] = {"certified": int(sum(c)), "wall_s": round(wall, 4),
+                         "certified_graphs_per_s": round(sum(c) / wall, 1)}
+        row["learned_over_plain"] = round(row["learned"]["certified_graphs_per_s"] /
+                                          max(row["plain"]["certified_graphs_per_s"], 1e-9), 3)
+        sweep.append(row)
+    ctx.set_param("bnb.node_budget", budget)
+    out["bnb1024_budget_sweep"] = sweep
     return out
 
 

@@ -208,12 +208,17 @@ __device__ int TreeRank(uint64_t myadj, uint64_t C, int* top, uint64_t* sL, int8
 }
 
 // Lower bound on td(C), C connected, all from proven facts.
+// Cheapest bound first: returns as soon as one exceeds k.  e = |E(C)|, or -1
+// to count it here.
 __device__ __forceinline__ int LowerBound(uint64_t myadj, bool in_c, uint64_t C,
-                                          int k) {
+                                          int k, int e = -1) {
   const int n = __builtin_popcountll(C);
-  const int deg = in_c ? __builtin_popcountll(myadj & C) : 0;
-  const int e = static_cast<int>(WaveSum(static_cast<uint32_t>(deg))) / 2;
+  if (e < 0) {
+    const int deg = in_c ? __builtin_popcountll(myadj & C) : 0;
+    e = static_cast<int>(WaveSum(static_cast<uint32_t>(deg))) / 2;
+  }
   int lb = EdgeLB(n, e);
+  if (lb > k) return lb;
   // path: double sweep BFS from the lowest vertex
   int ecc = 0;
   uint64_t last = 0;
@@ -492,7 +497,7 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
         --sp;
         continue;
       }
-      int lb = LowerBound(S.myadj, in_c, C, k);
+      int lb = LowerBound(S.myadj, in_c, C, k, ecount);
       if (lb <= k && ecount - (n - 1) <= 12) lb = max(lb, TreeRank(S.myadj, C, nullptr, S.tl, S.tp));
       int slot;
       const uint32_t md = MemoResolve(S.memo, C, me, &slot);
