@@ -295,8 +295,9 @@ def profile_traffic():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); "
-              "the multi-GPU design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): graphs would be "
+              "sharded one process per GPU, which this build does not implement",
+              file=sys.stderr)
         sys.exit(2)
     from gisbdw_amd import api
     from gisbdw_amd import graphs as G

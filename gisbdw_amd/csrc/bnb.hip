@@ -426,6 +426,7 @@ struct Search {
   const int* cancel;     // LDS flag set when another wave of the graph succeeded
   uint64_t* tl;          // TreeRank scratch (this wave's LDS)
   int8_t* tp;
+  bool building;         // inside Build: every decided set must leave a root
 };
 
 // A halted search (budget or cancellation) returns false without storing a
@@ -486,7 +487,14 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
       const int ecount =
           static_cast<int>(WaveSum(in_c ? __builtin_popcountll(S.myadj & C) : 0)) / 2;
       if (ecount == n - 1) {
-        // a tree: exact treedepth and an optimal root, no search
+        // a tree: td <= 1 + floor(log2 |C|), so k at least that decides it
+        // without ranking (Build, which needs the root, ranks it)
+        if (!S.building && k >= 32 - __builtin_clz(static_cast<uint32_t>(n))) {
+          ret = true;
+          --sp;
+          continue;
+        }
+        // exact treedepth and an optimal root, no search
         int top;
         const int t = TreeRank(S.myadj, C, &top, S.tl, S.tp);
         int slot;
@@ -498,7 +506,11 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
         continue;
       }
       int lb = LowerBound(S.myadj, in_c, C, k, ecount);
-      if (lb <= k && ecount - (n - 1) <= 12) lb = max(lb, TreeRank(S.myadj, C, nullptr, S.tl, S.tp));
+      // a spanning tree of C has treedepth <= 1 + floor(log2 |C|) (a centroid
+      // leaves parts of at most half the vertices), so its ranking can only
+      // exceed k when k is below that
+      if (lb <= k && ecount - (n - 1) <= 12 && k < 32 - __builtin_clz(static_cast<uint32_t>(n)))
+        lb = max(lb, TreeRank(S.myadj, C, nullptr, S.tl, S.tp));
       int slot;
       const uint32_t md = MemoResolve(S.memo, C, me, &slot);
       if (slot >= 0) {
@@ -659,7 +671,9 @@ __device__ void Build(Search& S, uint64_t C0, int k0, int par0, int8_t* parent,
     if (!(sk && sk <= static_cast<uint32_t>(k))) {
       const uint64_t saved_budget = S.budget;
       S.budget = 0;  // the set is known feasible: finish the search
+      S.building = true;
       Feasible(S, C, k);
+      S.building = false;
       S.budget = saved_budget;
       md = MemoProbe(S.memo, C, &slot);
       sk = (slot >= 0) ? (md >> 8) & 0xFF : 0;
@@ -803,6 +817,7 @@ __global__ __launch_bounds__(256) void BnBKernel(
   const int wid = threadIdx.x >> 6;
   const int W = blockDim.x >> 6;
   Search S;
+  S.building = false;
   S.st = s_stack[wid];
   S.tl = s_tl[wid];
   S.tp = s_tp[wid];
