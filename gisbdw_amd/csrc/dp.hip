@@ -106,7 +106,9 @@ __device__ __forceinline__ void StoreTile(uint8_t* p,
 // the k-subsets of its class, prev / cur = first tile of class p at levels
 // h-1 / h, and x[0..nx) = first tiles of the classes p \ b (b in p) at level
 // h-1, where the neighbour tile H \ b sits at the same rank t.
-template <int LB, bool kClass>
+// kTiny: h <= 1, the only levels with single-vertex subsets and the only
+// ones where a tile may have fewer than two neighbour tiles.
+template <int LB, bool kClass, bool kTiny>
 __device__ __forceinline__ void DPTile(const uint8_t* __restrict__ prev,
                                        uint8_t* __restrict__ cur, uint32_t t,
                                        uint32_t Hr, int k, uint32_t H, int h,
@@ -121,11 +123,13 @@ __device__ __forceinline__ void DPTile(const uint8_t* __restrict__ prev,
   // of each byte = "some S \ x is connected").
   u16x2 mnE[NW], mnO[NW], mxE[NW], mxO[NW];
   uint32_t orr[NW];
+  if constexpr (kTiny) {
 #pragma unroll
-  for (int w = 0; w < NW; ++w) {
-    mnE[w] = mnO[w] = AsU16x2(0x007F007Fu);
-    mxE[w] = mxO[w] = AsU16x2(0u);
-    orr[w] = 0u;
+    for (int w = 0; w < NW; ++w) {
+      mnE[w] = mnO[w] = AsU16x2(0x007F007Fu);
+      mxE[w] = mxO[w] = AsU16x2(0u);
+      orr[w] = 0u;
+    }
   }
   // neighbour tiles Hr \ y_j, j = k-1 .. 0 (y_j = j-th smallest of Hr):
   // rank(Hr \ y_j) = t - C(y_j, j+1) - sum_{i>j} (C(y_i, i+1) - C(y_i, i)),
@@ -163,11 +167,18 @@ __device__ __forceinline__ void DPTile(const uint8_t* __restrict__ prev,
           const uint32_t xw = wv[u][w];
           const u16x2 xe = AsU16x2(__builtin_amdgcn_perm(xw, xw, 0x0c020c00u));
           const u16x2 xo = AsU16x2(__builtin_amdgcn_perm(xw, xw, 0x0c030c01u));
-          mnE[w] = __builtin_elementwise_min(mnE[w], xe);
-          mnO[w] = __builtin_elementwise_min(mnO[w], xo);
-          mxE[w] = __builtin_elementwise_max(mxE[w], xe);
-          mxO[w] = __builtin_elementwise_max(mxO[w], xo);
-          orr[w] |= xw;
+          if (!kTiny && u == 0 && i0 == 0) {
+            // h >= 2: the first neighbour tile starts the fold
+            mnE[w] = mxE[w] = xe;
+            mnO[w] = mxO[w] = xo;
+            orr[w] = xw;
+          } else {
+            mnE[w] = __builtin_elementwise_min(mnE[w], xe);
+            mnO[w] = __builtin_elementwise_min(mnO[w], xo);
+            mxE[w] = __builtin_elementwise_max(mxE[w], xe);
+            mxO[w] = __builtin_elementwise_max(mxO[w], xo);
+            orr[w] |= xw;
+          }
         }
       }
     }
@@ -199,7 +210,7 @@ __device__ __forceinline__ void DPTile(const uint8_t* __restrict__ prev,
     const uint32_t cover = NH | nl[L];
     const bool conn = ((S & ~cover) == 0u) && (o & 1u);
     uint32_t val = conn ? ((m & ~1u) + 3u) : (M & ~1u);
-    if (__builtin_popcount(L) + h == 1) val = 3u;  // a single vertex
+    if (kTiny && __builtin_popcount(L) + h == 1) val = 3u;  // a single vertex
     V[L] = val;
   }
   uint32_t ow[NW];
@@ -244,7 +255,10 @@ __device__ __forceinline__ void DPLevelRange(const DPGraph* __restrict__ g,
     uint32_t H = Unrank(t0, h, nh, s_cb);
     const uint32_t tend = min(t0 + static_cast<uint32_t>(R), end);
     for (uint32_t t = t0;;) {
-      DPTile<LB, false>(prev, cur, t, H, h, H, h, none, 0, s_cb, s_nbh, nl);
+      if (h <= 1)
+        DPTile<LB, false, true>(prev, cur, t, H, h, H, h, none, 0, s_cb, s_nbh, nl);
+      else
+        DPTile<LB, false, false>(prev, cur, t, H, h, H, h, none, 0, s_cb, s_nbh, nl);
       if (++t >= tend) break;
       H = NextCombination(H);
     }
@@ -331,7 +345,10 @@ __global__ __launch_bounds__(1024) void DPClassLevelKernel(
       const uint32_t tend = min(t0 + static_cast<uint32_t>(R), sg.end);
       for (uint32_t t = t0;;) {
         const uint32_t H = p | (Hr << cb);
-        DPTile<LB, true>(prev, cur, t, Hr, k, H, h, x, np, s_cb, s_nbh, nl);
+        if (h <= 1)
+          DPTile<LB, true, true>(prev, cur, t, Hr, k, H, h, x, np, s_cb, s_nbh, nl);
+        else
+          DPTile<LB, true, false>(prev, cur, t, Hr, k, H, h, x, np, s_cb, s_nbh, nl);
         if (++t >= tend) break;
         Hr = NextCombination(Hr);
       }
