@@ -63,6 +63,9 @@ def parse():
                     help="keep the default launch shape instead of td_dp_autotune's pick")
     ap.add_argument("--no-bnb", action="store_true",
                     help="skip the branch-and-bound / learned-proposal extra figures")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the timed workload (no copy, B&B, other-configuration or CPU "
+                         "figures): the form used for kernel profiles")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (rocprof calibration)")
     return ap.parse_args()
@@ -448,11 +451,14 @@ def main():
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
             "extra": {"subsets_per_s": sub / wall},
         }
-    out["extra"]["copy_GBps_2GiB"] = ctx.copy_bench(2 << 30, 3)
+    if args.no_extra:
+        args.no_bnb = args.no_cpu = True
+    else:
+        out["extra"]["copy_GBps_2GiB"] = ctx.copy_bench(2 << 30, 3)
     if not args.no_bnb and args.workload == "dp30":
         print("[bench] branch-and-bound extra", file=sys.stderr, flush=True)
         out["extra"]["bnb"] = bnb_extra(ctx)
-    if args.workload == "dp30":
+    if args.workload == "dp30" and not args.no_extra:
         out["extra"]["gpu_configs"] = gpu_configs_extra(ctx)
     if not args.no_cpu:
         g0 = graphs[0]

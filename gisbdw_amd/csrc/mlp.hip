@@ -420,14 +420,36 @@ __global__ __launch_bounds__(64) void MLPTrainKernel(
   (void)gM;
 }
 
-// grad[p] = sum over rows of part[*][p], fixed order (deterministic)
-__global__ void MLPReduceKernel(const float* __restrict__ part, int rows,
-                                float* __restrict__ grad) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kNParams) return;
+// grad[p] = sum over rows of part[*][p] in a fixed order (deterministic):
+// a workgroup of 4 waves owns 64 parameters, wave w sums the rows r = w mod
+// 4 (one coalesced 256-byte row slice per step), then wave 0 adds the four
+// partial sums in wave order.
+constexpr int kReduceParams = 64;
+__global__ __launch_bounds__(256) void MLPReduceKernel(const float* __restrict__ part,
+                                                       int rows, float* __restrict__ grad) {
+  __shared__ float s_part[4][kReduceParams];
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int p = blockIdx.x * kReduceParams + lane;
   float s = 0.f;
-  for (int r = 0; r < rows; ++r) s += part[static_cast<size_t>(r) * kNParams + p];
-  grad[p] = s;
+  if (p < kNParams) {
+    int r = wid;
+    for (; r + 12 < rows; r += 16) {
+      const float a = part[static_cast<size_t>(r) * kNParams + p];
+      const float b = part[static_cast<size_t>(r + 4) * kNParams + p];
+      const float c = part[static_cast<size_t>(r + 8) * kNParams + p];
+      const float d = part[static_cast<size_t>(r + 12) * kNParams + p];
+      s += a;
+      s += b;
+      s += c;
+      s += d;
+    }
+    for (; r < rows; r += 4) s += part[static_cast<size_t>(r) * kNParams + p];
+  }
+  s_part[wid][lane] = s;
+  __syncthreads();
+  if (wid == 0 && p < kNParams)
+    grad[p] = ((s_part[0][lane] + s_part[1][lane]) + s_part[2][lane]) + s_part[3][lane];
 }
 
 __global__ void MLPAdamKernel(float* __restrict__ w, uint16_t* __restrict__ wb,
@@ -679,8 +701,8 @@ int td_mlp_loss_grad(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint
     hipLaunchKernelGGL(MLPTrainKernel, dim3(ngraphs), dim3(64), 0, ctx->stream,
                        Net{d.w, d.wb}, u.ns, u.adj, tg, u.feat, nullptr, scratch, scf, part,
                        loss, 1.0f / static_cast<float>(ngraphs));
-    hipLaunchKernelGGL(MLPReduceKernel, dim3((kNParams + 255) / 256), dim3(256), 0,
-                       ctx->stream, part, ngraphs, grad);
+    hipLaunchKernelGGL(MLPReduceKernel, dim3((kNParams + kReduceParams - 1) / kReduceParams),
+                       dim3(256), 0, ctx->stream, part, ngraphs, grad);
     TD_HIP(hipGetLastError());
     std::vector<float> l(ngraphs);
     TD_HIP(hipMemcpyAsync(l.data(), loss, 4ull * ngraphs, hipMemcpyDeviceToHost, ctx->stream));
@@ -737,7 +759,8 @@ int td_mlp_train(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint64_t
         hipLaunchKernelGGL(MLPTrainKernel, dim3(batch), dim3(64), 0, ctx->stream,
                            Net{d.w, d.wb}, u.ns, u.adj, tg, u.feat, idx + s0, scratch, scf,
                            part, loss + s0, 1.0f / static_cast<float>(batch));
-        hipLaunchKernelGGL(MLPReduceKernel, dim3((kNParams + 255) / 256), dim3(256), 0,
+        hipLaunchKernelGGL(MLPReduceKernel,
+                           dim3((kNParams + kReduceParams - 1) / kReduceParams), dim3(256), 0,
                            ctx->stream, part, batch, grad);
         ++t;
         const float c1 = 1.f - std::pow(b1, static_cast<float>(t));
