@@ -140,3 +140,37 @@ def test_bnb_trees_exact(ctx, oracle):
     for g, t, p, c in zip(gs, tds, pars, cert):
         assert c and t == oracle.dp_fast(g)[0], g.name
         assert oracle.certify(g, p, t)
+
+
+def test_bnb_closed_forms_at_the_size_limit(ctx, oracle):
+    """Graphs up to n = 64 whose treedepth has a closed form: complete graphs,
+    stars, edgeless graphs, paths, cycles, complete bipartite graphs and a
+    union of cliques.  All certified, all trees pass the certifier."""
+    import math
+
+    cases = []
+    for n in (1, 2, 40, 63, 64):
+        cases.append((G.complete(n), n))
+    for n in (2, 33, 64):
+        cases.append((G.star(n), 1 if n == 1 else 2))
+    for n in (1, 64):
+        cases.append((G.edgeless(n), 1))
+    for n in (33, 63, 64):
+        cases.append((G.path(n), math.ceil(math.log2(n + 1))))
+    for n in (33, 64):
+        cases.append((G.cycle(n), 1 + math.ceil(math.log2(n))))
+    for a, b in ((5, 40), (1, 63), (32, 32)):
+        cases.append((G.complete_bipartite(a, b), min(a, b) + 1))
+    # disjoint cliques K_1 .. K_10 (55 vertices): td = 10
+    adj, off = [], 0
+    for s in range(1, 11):
+        for v in range(s):
+            adj.append(sum(1 << (off + u) for u in range(s) if u != v))
+        off += s
+    cases.append((G.Graph(off, adj, "cliques1-10"), 10))
+    graphs = [g for g, _ in cases]
+    tds, pars, cert, st = api.bnb_solve_batch(ctx, graphs)
+    for (g, want), t, p, c in zip(cases, tds, pars, cert):
+        print("closed form", g.name, g.n, want, t, c, flush=True)
+        assert c and t == want, g.name
+        assert oracle.certify(g, p, t), g.name
