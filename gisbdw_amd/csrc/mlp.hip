@@ -35,12 +35,12 @@ constexpr int P_W0 = 0, P_B0 = 256, P_WS1 = 320, P_WN1 = 4416, P_B1 = 8512,
               P_WS2 = 8576, P_WN2 = 12672, P_B2 = 16768, P_W3 = 16832,
               P_B3 = 20928, P_W4 = 20992, P_B4 = 25088, P_W5 = 25152,
               P_B5 = 25216, kNParams = 25217;
-// per-graph scratch: 7 activation matrices + 4 gradient matrices (bf16,
-// 64x64 each) + 2 fp32 matrices
+// per-graph working set, in LDS: 7 activation matrices and, for training,
+// one bf16 gradient matrix (64x64 each) and two fp32 matrices: 56 KiB for
+// scoring, 96 KiB for training (gfx950 has 160 KiB of LDS per CU)
 constexpr int kMat = kH * kH;
-enum { A_H0 = 0, A_M1, A_H1, A_M2, A_H2, A_Z1, A_Z2, G_A, G_B, G_M, G_T, kNBf16 };
-constexpr size_t kScratchBf16 = static_cast<size_t>(kNBf16) * kMat;  // uint16
-constexpr size_t kScratchF32 = 2 * kMat;
+enum { A_H0 = 0, A_M1, A_H1, A_M2, A_H2, A_Z1, A_Z2, G_A, kNBf16 };
+constexpr int kScoreMats = A_Z2 + 1;
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef short v8s __attribute__((ext_vector_type(8)));
@@ -152,7 +152,7 @@ struct Net {
 
 __device__ __forceinline__ float Wb(const Net& N, int p) { return FromBf16(N.wb[p]); }
 
-// Forward for one graph (this wave); activations to scratch; returns s_v in
+// Forward for one graph (this wave); activations to S (LDS); returns s_v in
 // lane v.
 __device__ float Forward(const Net& N, const float* __restrict__ x, uint64_t myadj,
                          uint16_t* S) {
@@ -272,13 +272,12 @@ __global__ __launch_bounds__(64) void MLPFeatureKernel(const int32_t* __restrict
 __global__ __launch_bounds__(64) void MLPScoreKernel(Net N, const int32_t* __restrict__ ns,
                                                      const uint64_t* __restrict__ adj,
                                                      const float* __restrict__ feat,
-                                                     uint16_t* __restrict__ scratch,
                                                      float* __restrict__ scores) {
+  __shared__ uint16_t s_act[kScoreMats * kMat];
   const int gi = blockIdx.x, v = threadIdx.x;
   const int n = ns[gi];
   const uint64_t myadj = v < n ? adj[static_cast<size_t>(gi) * 64 + v] : 0ull;
-  const float s = Forward(N, feat + static_cast<size_t>(gi) * 256, myadj,
-                          scratch + static_cast<size_t>(blockIdx.x) * kScratchBf16);
+  const float s = Forward(N, feat + static_cast<size_t>(gi) * 256, myadj, s_act);
   scores[static_cast<size_t>(gi) * 64 + v] = v < n ? s : 0.f;
 }
 
@@ -287,8 +286,7 @@ __global__ __launch_bounds__(64) void MLPScoreKernel(Net N, const int32_t* __res
 __global__ __launch_bounds__(64) void MLPTrainKernel(
     Net N, const int32_t* __restrict__ ns, const uint64_t* __restrict__ adj,
     const uint64_t* __restrict__ targets, const float* __restrict__ feat,
-    const int32_t* __restrict__ index, uint16_t* __restrict__ scratch,
-    float* __restrict__ scratchf, float* __restrict__ part, float* __restrict__ loss,
+    const int32_t* __restrict__ index, float* __restrict__ part, float* __restrict__ loss,
     float scale) {
   const int bi = blockIdx.x;
   const int gi = index ? index[bi] : bi;
@@ -296,8 +294,10 @@ __global__ __launch_bounds__(64) void MLPTrainKernel(
   const int n = ns[gi];
   const bool in = v < n;
   const uint64_t myadj = in ? adj[static_cast<size_t>(gi) * 64 + v] : 0ull;
-  uint16_t* S = scratch + static_cast<size_t>(bi) * kScratchBf16;
-  float* SF = scratchf + static_cast<size_t>(bi) * kScratchF32;
+  __shared__ uint16_t s_act[kNBf16 * kMat];
+  __shared__ float s_f[2 * kMat];
+  uint16_t* S = s_act;
+  float* SF = s_f;
   float* G = part + static_cast<size_t>(bi) * kNParams;
   const float* x = feat + static_cast<size_t>(gi) * 256;
   const float s = Forward(N, x, myadj, S);
@@ -321,7 +321,6 @@ __global__ __launch_bounds__(64) void MLPTrainKernel(
   uint16_t* z1 = S + A_Z1 * kMat;
   uint16_t* z2 = S + A_Z2 * kMat;
   uint16_t* gA = S + G_A * kMat;  // bf16 gradient (pre-activation)
-  uint16_t* gM = S + G_M * kMat;  // bf16 gradient w.r.t. an aggregate
   float* F0 = SF;                 // fp32 gradient w.r.t. an activation
   float* F1 = SF + kMat;
   // ---- head: s = w5 . z2 + b5
@@ -417,7 +416,6 @@ __global__ __launch_bounds__(64) void MLPTrainKernel(
     for (int i = 0; i < 4; ++i) G[P_W0 + v * 4 + i] = dw[i];
     G[P_B0 + v] = db;
   }
-  (void)gM;
 }
 
 // grad[p] = sum over rows of part[*][p] in a fixed order (deterministic):
@@ -663,11 +661,9 @@ int td_mlp_scores(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint64_
     CheckGraphs(ngraphs, ns, adj64);
     DevNet d = NetBuffers(ctx);
     Uploaded u = UploadGraphs(ctx, "mlp.in", ngraphs, ns, adj64);
-    auto* scratch = static_cast<uint16_t*>(
-        ctx->Buffer("mlp.scratch", kScratchBf16 * 2 * static_cast<size_t>(ngraphs)));
     auto* sc = static_cast<float*>(ctx->Buffer("mlp.scores", 256ull * ngraphs));
     hipLaunchKernelGGL(MLPScoreKernel, dim3(ngraphs), dim3(64), 0, ctx->stream,
-                       Net{d.w, d.wb}, u.ns, u.adj, u.feat, scratch, sc);
+                       Net{d.w, d.wb}, u.ns, u.adj, u.feat, sc);
     TD_HIP(hipGetLastError());
     TD_HIP(hipMemcpyAsync(scores_out, sc, 256ull * ngraphs, hipMemcpyDeviceToHost,
                           ctx->stream));
@@ -690,17 +686,13 @@ int td_mlp_loss_grad(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint
     Uploaded u = UploadGraphs(ctx, "mlp.in", ngraphs, ns, adj64);
     auto* tg = static_cast<uint64_t*>(ctx->Buffer("mlp.in.tgt", 8ull * ngraphs));
     TD_HIP(hipMemcpyAsync(tg, targets, 8ull * ngraphs, hipMemcpyHostToDevice, ctx->stream));
-    auto* scratch = static_cast<uint16_t*>(
-        ctx->Buffer("mlp.scratch", kScratchBf16 * 2 * static_cast<size_t>(ngraphs)));
-    auto* scf = static_cast<float*>(
-        ctx->Buffer("mlp.scratchf", kScratchF32 * 4 * static_cast<size_t>(ngraphs)));
     auto* part = static_cast<float*>(
         ctx->Buffer("mlp.part", 4ull * kNParams * static_cast<size_t>(ngraphs)));
     auto* loss = static_cast<float*>(ctx->Buffer("mlp.loss", 4ull * ngraphs));
     auto* grad = static_cast<float*>(ctx->Buffer("mlp.grad", 4ull * kNParams));
     hipLaunchKernelGGL(MLPTrainKernel, dim3(ngraphs), dim3(64), 0, ctx->stream,
-                       Net{d.w, d.wb}, u.ns, u.adj, tg, u.feat, nullptr, scratch, scf, part,
-                       loss, 1.0f / static_cast<float>(ngraphs));
+                       Net{d.w, d.wb}, u.ns, u.adj, tg, u.feat, nullptr, part, loss,
+                       1.0f / static_cast<float>(ngraphs));
     hipLaunchKernelGGL(MLPReduceKernel, dim3((kNParams + kReduceParams - 1) / kReduceParams),
                        dim3(256), 0, ctx->stream, part, ngraphs, grad);
     TD_HIP(hipGetLastError());
@@ -735,10 +727,6 @@ int td_mlp_train(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint64_t
     Uploaded u = UploadGraphs(ctx, "mlp.train", ngraphs, ns, adj64);
     auto* tg = static_cast<uint64_t*>(ctx->Buffer("mlp.train.tgt", 8ull * ngraphs));
     TD_HIP(hipMemcpyAsync(tg, targets, 8ull * ngraphs, hipMemcpyHostToDevice, ctx->stream));
-    auto* scratch = static_cast<uint16_t*>(
-        ctx->Buffer("mlp.scratch", kScratchBf16 * 2 * static_cast<size_t>(batch)));
-    auto* scf = static_cast<float*>(
-        ctx->Buffer("mlp.scratchf", kScratchF32 * 4 * static_cast<size_t>(batch)));
     auto* part = static_cast<float*>(
         ctx->Buffer("mlp.part", 4ull * kNParams * static_cast<size_t>(batch)));
     auto* loss = static_cast<float*>(
@@ -757,8 +745,8 @@ int td_mlp_train(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint64_t
       int nb = 0;
       for (int s0 = 0; s0 + batch <= ngraphs; s0 += batch, ++nb) {
         hipLaunchKernelGGL(MLPTrainKernel, dim3(batch), dim3(64), 0, ctx->stream,
-                           Net{d.w, d.wb}, u.ns, u.adj, tg, u.feat, idx + s0, scratch, scf,
-                           part, loss + s0, 1.0f / static_cast<float>(batch));
+                           Net{d.w, d.wb}, u.ns, u.adj, tg, u.feat, idx + s0, part, loss + s0,
+                           1.0f / static_cast<float>(batch));
         hipLaunchKernelGGL(MLPReduceKernel,
                            dim3((kNParams + kReduceParams - 1) / kReduceParams), dim3(256), 0,
                            ctx->stream, part, batch, grad);
