@@ -210,8 +210,10 @@ __device__ int TreeRank(uint64_t myadj, uint64_t C, int* top, uint64_t* sL, int8
 // Lower bound on td(C), C connected, all from proven facts.
 // Cheapest bound first: returns as soon as one exceeds k.  e = |E(C)|, or -1
 // to count it here.
+// *height (if non-null) receives the eccentricity of C's lowest vertex, the
+// height of the BFS tree TreeRank ranks (0 when the BFS was not reached).
 __device__ __forceinline__ int LowerBound(uint64_t myadj, bool in_c, uint64_t C,
-                                          int k, int e = -1) {
+                                          int k, int e = -1, int* height = nullptr) {
   const int n = __builtin_popcountll(C);
   if (e < 0) {
     const int deg = in_c ? __builtin_popcountll(myadj & C) : 0;
@@ -223,6 +225,7 @@ __device__ __forceinline__ int LowerBound(uint64_t myadj, bool in_c, uint64_t C,
   int ecc = 0;
   uint64_t last = 0;
   Component(myadj, C, C & (0ull - C), &ecc, &last);
+  if (height) *height = ecc;
   Component(myadj, C, last & (0ull - last), &ecc, nullptr);
   lb = max(lb, CeilLog2(ecc + 2));
   if (lb > k) return lb;
@@ -505,11 +508,13 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
         --sp;
         continue;
       }
-      int lb = LowerBound(S.myadj, in_c, C, k, ecount);
-      // a spanning tree of C has treedepth <= 1 + floor(log2 |C|) (a centroid
-      // leaves parts of at most half the vertices), so its ranking can only
-      // exceed k when k is below that
-      if (lb <= k && ecount - (n - 1) <= 12 && k < 32 - __builtin_clz(static_cast<uint32_t>(n)))
+      int height = 0;
+      int lb = LowerBound(S.myadj, in_c, C, k, ecount, &height);
+      // the BFS spanning tree of C has treedepth <= 1 + floor(log2 |C|) (a
+      // centroid leaves parts of at most half the vertices) and <= its height
+      // + 1, so its ranking can only exceed k when k is below both
+      if (lb <= k && ecount - (n - 1) <= 12 && k < 32 - __builtin_clz(static_cast<uint32_t>(n)) &&
+          k <= height)
         lb = max(lb, TreeRank(S.myadj, C, nullptr, S.tl, S.tp));
       int slot;
       const uint32_t md = MemoResolve(S.memo, C, me, &slot);
