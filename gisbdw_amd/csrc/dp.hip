@@ -450,34 +450,35 @@ __global__ __launch_bounds__(64) void DPWalkKernel(
   __syncthreads();
   const DPGraph* g = gs + blockIdx.x;
   const int lane = threadIdx.x;
+  const uint32_t myadj = lane < 32 ? g->adj[lane] : 0u;
   uint32_t S = start[blockIdx.x];
   const int steps = nsteps[blockIdx.x];
   int32_t* out = orders + static_cast<size_t>(blockIdx.x) * order_stride;
   for (int step = 0; step < steps && S != 0u; ++step) {
-    // components of S, lowest vertex first (uniform scalar work)
-    uint32_t rest = S, mine = 0;
-    int ncomp = 0;
+    // component of every vertex of S, by ballot BFS from each lowest vertex
+    const bool in = lane < 32 && ((S >> lane) & 1u);
+    uint32_t rest = S, mine = 0u;
     while (rest) {
-      uint32_t comp = rest & (0u - rest), frontier = comp;
-      while (frontier) {
-        const int u = __builtin_ctz(frontier);
-        frontier &= frontier - 1;
-        const uint32_t nb = g->adj[u] & rest & ~comp;
+      uint32_t comp = rest & (0u - rest), front = comp;
+      for (;;) {
+        const uint32_t nb =
+            static_cast<uint32_t>(__ballot((myadj & front) != 0u)) & rest & ~comp;
+        if (!nb) break;
         comp |= nb;
-        frontier |= nb;
+        front = nb;
       }
-      if (lane == ncomp) mine = comp;
+      if ((comp >> lane) & 1u) mine = comp;
       rest &= ~comp;
-      ++ncomp;
     }
-    const uint32_t tdc = lane < ncomp ? Lookup(g, arena, s_cb, mine) : 0u;
+    // one round of lookups: td of v's component and of that component minus v
+    const uint32_t tdc = in ? Lookup(g, arena, s_cb, mine) : 0u;
+    const uint32_t tv = in ? Lookup(g, arena, s_cb, mine & ~(1u << lane)) : 0xFFu;
     const uint32_t T = WaveMax(tdc);
-    const unsigned long long cb = __ballot(lane < ncomp && tdc == T);
-    const int ci = __ffsll(static_cast<long long>(cb)) - 1;
-    const uint32_t C = __shfl(mine, ci);
-    const bool in = lane < 32 && (C >> lane & 1u);
-    const uint32_t tv = in ? Lookup(g, arena, s_cb, C & ~(1u << lane)) : 0xFFu;
-    const unsigned long long rb = __ballot(in && tv + 1u == T);
+    // the first component (lowest vertex order) of largest td holds the
+    // lowest vertex whose component has td T
+    const unsigned long long cb = __ballot(in && tdc == T);
+    const uint32_t C = __shfl(mine, __ffsll(static_cast<long long>(cb)) - 1);
+    const unsigned long long rb = __ballot(in && ((C >> lane) & 1u) && tv + 1u == T);
     const int root = __ffsll(static_cast<long long>(rb)) - 1;
     if (lane == 0) out[step] = root;
     S &= ~(1u << root);
