@@ -112,6 +112,27 @@ def bnb_extra(ctx, budget=50000):
     t0 = time.perf_counter()
     tds, pars, cert, st = api.bnb_solve_batch(ctx, W)
     wall_p = time.perf_counter() - t0
+    heur_gpu, _, _ = api.bnb_last_details(ctx, len(W))
+    # the reference's heuristic chain HighestDegree(SpanningTree(Variance)) on
+    # the GPU, as an alternative upper-bound seed
+    t0 = time.perf_counter()
+    chain_td, _ = api.heuristic_batch(ctx, W, "chain")
+    chain_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tdc, _, certc, _ = api.bnb_solve_batch(ctx, W, ub_seed=chain_td)
+    wall_c = time.perf_counter() - t0 + chain_s
+    cert_td = [t for t, c in zip(tds, cert) if c]
+    out["ub_seeds"] = {
+        "reference_chain_mean_depth": sum(chain_td) / len(W),
+        "reference_chain_ms_1024_graphs": chain_s * 1e3,
+        "gpu_seed_trees_mean_depth": float(sum(heur_gpu)) / len(W),
+        "certified_graphs": len(cert_td),
+        "certified_mean_td": sum(cert_td) / max(len(cert_td), 1),
+        "reference_chain_mean_depth_on_certified": sum(h for h, c in zip(chain_td, cert) if c) / max(len(cert_td), 1),
+        "gpu_seed_mean_depth_on_certified": float(sum(h for h, c in zip(heur_gpu, cert) if c)) / max(len(cert_td), 1),
+        "bnb_with_reference_chain_seed": {"certified": int(sum(certc)), "wall_s": wall_c,
+                                          "certified_graphs_per_s": sum(certc) / wall_c},
+    }
     t0 = time.perf_counter()
     scores = prop.scores(W)
     tdl, parl, certl, stl = api.bnb_solve_batch(ctx, W, scores=scores)

@@ -100,6 +100,8 @@ def load_library():
         "td_dp_autotune_batch": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
                                            C.c_char_p, P(C.c_double)]),
         "td_ctx_copy_bench": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, P(C.c_double)]),
+        "td_heuristic_batch": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
+                                         C.c_int32, P(C.c_int32), P(C.c_int8)]),
     }
     sig.update({
         "td_bnb_solve_batch": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
@@ -295,6 +297,27 @@ def dp_solve_batch_arrays(ctx: Context, ns: np.ndarray, adj: np.ndarray, mode: i
                                         _ptr(adj, C.c_uint64), _ptr(td, C.c_uint8),
                                         _ptr(par, C.c_int8), mode, C.byref(st)))
     return td, par, st
+
+
+HEURISTIC_KINDS = {"chain": 1, "highest_degree": 2, "spanning_tree": 3, "variance": 4}
+
+
+def heuristic_batch(ctx: Context, graphs: Sequence[Graph], kind="chain"):
+    """td_heuristic_batch: the reference's upper-bound heuristics on the GPU.
+    kind: "chain" (HighestDegree(SpanningTree(Variance))), "highest_degree",
+    "spanning_tree" or "variance".  Returns (depths, parent lists)."""
+    ns, adj = pack_graphs(graphs)
+    td = np.zeros(len(ns), np.int32)
+    par = np.zeros(int(ns.sum()), np.int8)
+    k = HEURISTIC_KINDS[kind] if isinstance(kind, str) else int(kind)
+    ctx.check(ctx.lib.td_heuristic_batch(ctx.h, len(ns), _ptr(ns, C.c_int32),
+                                         _ptr(adj, C.c_uint64), k, _ptr(td, C.c_int32),
+                                         _ptr(par, C.c_int8)))
+    parents, off = [], 0
+    for g in graphs:
+        parents.append(par[off: off + g.n].astype(int).tolist())
+        off += g.n
+    return td.astype(int).tolist(), parents
 
 
 def dp_autotune_batch(ctx: Context, graphs: Sequence[Graph], cache_path=None) -> float:

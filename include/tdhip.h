@@ -130,6 +130,24 @@ int td_dp_autotune_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
                          double* best_ms_out);
 
 /* ---------------------------------------------------------------------- */
+/* Upper-bound heuristics                                                  */
+/* ---------------------------------------------------------------------- */
+/* The reference's elimination-tree heuristics on the GPU, one graph per
+ * wave, for ngraphs graphs of n[i] <= 64 vertices (rows concatenated):
+ *   kind 1: HighestDegree(SpanningTree(Variance(1.0, 0.2, 0.8))), the
+ *           decorator chain keeping an inner result only when strictly
+ *           shallower (the heuristics/ headers; BranchAndBound's Heuristic::Get,
+ *           Treedepth/src/branch_and_bound/branch_and_bound.hpp:38-42);
+ *   kind 2: HighestDegreeHeuristic::Get (highest_degree_heuristic.hpp:19-45);
+ *   kind 3: SpanningTreeHeuristic::Get (spanning_tree_heuristic.hpp:20-83);
+ *   kind 4: VarianceHeuristic::Get (variance_heuristic.hpp:24-78).
+ * td_out[i] = depth of the tree, parent_concat (nullable) its parent array
+ * (-1 = root).  Upper bounds, not proofs. */
+int td_heuristic_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
+                       const uint64_t* adj_concat, int32_t kind, int32_t* td_out,
+                       int8_t* parent_concat);
+
+/* ---------------------------------------------------------------------- */
 /* Branch-and-bound                                                        */
 /* ---------------------------------------------------------------------- */
 typedef struct td_bnb_stats {

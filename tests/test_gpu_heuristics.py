@@ -1,0 +1,48 @@
+"""GPU upper-bound heuristics (td_heuristic_batch in include/tdhip.h) against
+the oracle's restatements of the reference's HighestDegree, SpanningTree and
+Variance heuristics and their decorator chain: the same depth and the same
+parent array, bit for bit, on connected and disconnected graphs up to
+n = 64.  The Variance score is floating point; the oracle is built without
+fused multiply-add and the kernel computes it with contraction off, so the
+two must make the same choices."""
+import random
+
+import pytest
+
+from gisbdw_amd import api
+from gisbdw_amd import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+KINDS = {"chain": 1, "highest_degree": 2, "spanning_tree": 3, "variance": 4}
+
+
+def family():
+    rng = random.Random(2024)
+    gs = [G.path(1), G.path(2), G.path(33), G.cycle(17), G.star(20), G.complete(9),
+          G.complete_bipartite(4, 7), G.edgeless(5), G.grid(5, 6), G.grid(8, 8)]
+    gs += [G.gnp(rng.randint(3, 40), rng.uniform(0.05, 0.5), rng.getrandbits(32))
+           for _ in range(60)]
+    gs += [G.random_cubic(n, rng.getrandbits(32)) for n in (20, 40, 64)]
+    gs += [G.tree_plus_edges(rng.randint(20, 64), 0.1, rng.getrandbits(32)) for _ in range(20)]
+    gs += G.workload_bnb(24, 5)
+    return gs
+
+
+@pytest.mark.parametrize("kind", list(KINDS))
+def test_heuristics_equal_oracle(ctx, oracle, kind):
+    gs = family()
+    tds, pars = api.heuristic_batch(ctx, gs, kind)
+    for g, t, p in zip(gs, tds, pars):
+        want_td, want_par, _ = oracle.heuristic(g, KINDS[kind])
+        print(kind, g.name, g.n, t, want_td, flush=True)
+        assert t == want_td, (kind, g.name)
+        assert p == want_par, (kind, g.name)
+        assert oracle.certify(g, p, t), (kind, g.name)
+
+
+def test_heuristic_errors(ctx):
+    with pytest.raises(api.TDError):
+        api.heuristic_batch(ctx, [G.path(5)], 7)
+    with pytest.raises(api.TDError):
+        api.heuristic_batch(ctx, [G.Graph(3, [0b010, 0, 0])], 2)
