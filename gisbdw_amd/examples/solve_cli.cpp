@@ -1,8 +1,9 @@
-// solve_cli.cpp -- the reference CLI's dynGPU / bnbGPU paths
-// (Treedepth/main.cpp:178-195) on libtdhip.so through include/tdhip.hpp.
+// solve_cli.cpp -- the reference CLI's dynGPU / bnbGPU paths and its
+// highestDegreeHeur / spanningTreeHeur / varianceHeur options
+// (Treedepth/main.cpp:178-202) on libtdhip.so through include/tdhip.hpp.
 // Reads the reference's graph text ("N;" vertex lines, "a -- b" edges) and
 // prints treedepth, root and the decomposition's edges.
-//   solve_cli -a dynGPU|bnbGPU <graph file>
+//   solve_cli -a dynGPU|bnbGPU|highestDegreeHeur|spanningTreeHeur|varianceHeur <graph file>
 #include <fstream>
 #include <iostream>
 #include <regex>
@@ -13,7 +14,8 @@
 
 int main(int argc, char** argv) {
   if (argc != 4 || std::string(argv[1]) != "-a") {
-    std::cerr << "usage: solve_cli -a dynGPU|bnbGPU <graph file>\n";
+    std::cerr << "usage: solve_cli -a dynGPU|bnbGPU|highestDegreeHeur|spanningTreeHeur|"
+                 "varianceHeur <graph file>\n";
     return 2;
   }
   const std::string alg = argv[2];
@@ -45,6 +47,13 @@ int main(int argc, char** argv) {
     } else if (alg == "bnbGPU") {
       td::BnBGPU bnb(ctx);
       std::tie(r, certified) = bnb.Solve(g);
+    } else if (alg == "highestDegreeHeur" || alg == "spanningTreeHeur" ||
+               alg == "varianceHeur") {
+      const auto kind = alg == "highestDegreeHeur"  ? td::HeuristicGPU::kHighestDegree
+                        : alg == "spanningTreeHeur" ? td::HeuristicGPU::kSpanningTree
+                                                    : td::HeuristicGPU::kVariance;
+      r = td::HeuristicGPU(kind, ctx).Get(g);
+      certified = false;  // an upper bound
     } else {
       std::cerr << "Wrong algorithm option specified.\n";
       return 2;

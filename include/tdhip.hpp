@@ -167,6 +167,29 @@ class BnBGPU {
   bool certified_ = false;
 };
 
+// The reference's Heuristic plugins (Treedepth/src/branch_and_bound/
+// heuristics/) on the GPU: Get(g) like BranchAndBound::Heuristic::Get
+// (branch_and_bound.hpp:38-42).
+class HeuristicGPU {
+ public:
+  enum Kind { kChain = 1, kHighestDegree = 2, kSpanningTree = 3, kVariance = 4 };
+  explicit HeuristicGPU(Kind kind = kChain,
+                        std::shared_ptr<Context> ctx = std::make_shared<Context>())
+      : kind_(kind), ctx_(std::move(ctx)) {}
+
+  Result Get(const Graph& g) {
+    int32_t n = g.n, td = 0;
+    std::vector<int8_t> par(static_cast<std::size_t>(g.n));
+    ctx_->Check(td_heuristic_batch(ctx_->get(), 1, &n, g.adj.data(), kind_, &td, par.data()));
+    return Result::FromParent(std::vector<int>(par.begin(), par.end()),
+                              static_cast<unsigned>(td));
+  }
+
+ private:
+  Kind kind_;
+  std::shared_ptr<Context> ctx_;
+};
+
 }  // namespace td
 
 #endif  // TDHIP_HPP_

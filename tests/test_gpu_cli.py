@@ -31,3 +31,15 @@ def test_cli_p3_and_cycle(alg, tmp_path, oracle):
     g = G.cycle(11)
     out, edges = run(alg, g, tmp_path)
     assert int(out["Treedepth"]) == oracle.dp_fast(g)[0]
+
+
+@pytest.mark.parametrize("alg,kind", [("highestDegreeHeur", 2), ("spanningTreeHeur", 3),
+                                      ("varianceHeur", 4)])
+def test_cli_heuristics(alg, kind, tmp_path, oracle):
+    """The reference CLI's heuristic options: the same depth and tree as the
+    oracle's restatement of that heuristic, reported as uncertified."""
+    for g in (G.cycle(11), G.grid(3, 4), G.gnp_connected(14, 0.3, 2)):
+        out, edges = run(alg, g, tmp_path)
+        td, parent, _ = oracle.heuristic(g, kind)
+        assert int(out["Treedepth"]) == td and out["Certified"] == "0", g.name
+        assert sorted(edges) == sorted((v, p) for v, p in enumerate(parent) if p >= 0), g.name
