@@ -67,6 +67,19 @@ void ValidateGraph(int n, const uint64_t* adj, int max_n) {
 
 }  // namespace tdhip
 
+void* td_ctx::Pinned(size_t bytes) {
+  if (pinned && pinned_bytes >= bytes) return pinned;
+  if (pinned) {
+    TD_HIP(hipStreamSynchronize(stream));
+    TD_HIP(hipHostFree(pinned));
+    pinned = nullptr;
+    pinned_bytes = 0;
+  }
+  TD_HIP(hipHostMalloc(&pinned, bytes, hipHostMallocDefault));
+  pinned_bytes = bytes;
+  return pinned;
+}
+
 void* td_ctx::Buffer(const std::string& name, size_t bytes) {
   auto& b = bufs[name];
   if (b.bytes >= bytes && b.p) return b.p;
@@ -182,6 +195,7 @@ void td_ctx_destroy(td_ctx* c) {
   for (auto& kv : c->dp_graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto& kv : c->bufs)
     if (kv.second.p) (void)hipFree(kv.second.p);
+  if (c->pinned) (void)hipHostFree(c->pinned);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -111,6 +111,11 @@ struct td_ctx {
   std::map<std::string, tdhip::DeviceBuffer> bufs;
   void* Buffer(const std::string& name, size_t bytes);
   void Release(const std::string& name);
+  // page-locked host staging for the small per-call copies (grown on
+  // demand; every call synchronizes before returning, so it is reused)
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+  void* Pinned(size_t bytes);
 
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 

@@ -485,6 +485,61 @@ __global__ __launch_bounds__(64) void DPWalkKernel(
   }
 }
 
+// The canonical decomposition of one start set per wave, level by level of
+// the tree: every component of the vertices left takes its root (the
+// smallest v with td(C \ v) = td(C) - 1) in the same round, the parent of a
+// root is the root whose removal created its component, and all roots leave
+// together.  The tree is the one GetElimination's order produces (each
+// component's root depends only on the component), in about td rounds
+// instead of n.  parent_out: 64 int8 per graph; td_out: the tree's depth.
+__global__ __launch_bounds__(64) void DPTreeKernel(
+    const DPGraph* __restrict__ gs, const uint32_t* __restrict__ g_cb,
+    const uint8_t* __restrict__ arena, const uint32_t* __restrict__ start,
+    int8_t* __restrict__ parent_out, int32_t* __restrict__ td_out) {
+  __shared__ uint32_t s_cb[32 * kCbStride];
+  for (int i = threadIdx.x; i < 32 * kCbStride; i += blockDim.x)
+    s_cb[i] = g_cb[i];
+  __syncthreads();
+  const DPGraph* g = gs + blockIdx.x;
+  const int lane = threadIdx.x;
+  const uint32_t myadj = lane < 32 ? g->adj[lane] : 0u;
+  uint32_t S = start[blockIdx.x];
+  int creator = -1, depth = 0, parent = -1;
+  while (S) {
+    const bool in = lane < 32 && ((S >> lane) & 1u);
+    uint32_t rest = S, mine = 0u;
+    while (rest) {
+      uint32_t comp = rest & (0u - rest), front = comp;
+      for (;;) {
+        const uint32_t nb =
+            static_cast<uint32_t>(__ballot((myadj & front) != 0u)) & rest & ~comp;
+        if (!nb) break;
+        comp |= nb;
+        front = nb;
+      }
+      if ((comp >> lane) & 1u) mine = comp;
+      rest &= ~comp;
+    }
+    const uint32_t tdc = in ? Lookup(g, arena, s_cb, mine) : 0u;
+    const uint32_t tv = in ? Lookup(g, arena, s_cb, mine & ~(1u << lane)) : 0xFFu;
+    const uint32_t cand = static_cast<uint32_t>(__ballot(in && tv + 1u == tdc));
+    const uint32_t rc = cand & mine;
+    const int root = in ? __builtin_ctz(rc) : 0;  // every component has a candidate
+    const int cdepth = __shfl(depth, creator >= 0 ? creator : lane);
+    if (in && root == lane) {
+      parent = creator;
+      depth = creator >= 0 ? cdepth + 1 : 1;
+    } else if (in) {
+      creator = root;
+    }
+    S &= ~static_cast<uint32_t>(__ballot(in && root == lane));
+  }
+  int8_t* out = parent_out + static_cast<size_t>(blockIdx.x) * 64;
+  if (lane < 32 && ((start[blockIdx.x] >> lane) & 1u)) out[lane] = static_cast<int8_t>(parent);
+  const uint32_t d = WaveMax(static_cast<uint32_t>(depth));
+  if (lane == 0) td_out[blockIdx.x] = static_cast<int32_t>(d);
+}
+
 // Level-k table in colex order over n vertices (td values).
 __global__ __launch_bounds__(256) void DPExportLevelKernel(
     const DPGraph* __restrict__ g, const uint32_t* __restrict__ g_cb,
@@ -711,47 +766,6 @@ void RunLevels(td_ctx* ctx, const DPGraph& G, const DPGraph* dG,
   ++*launches;
 }
 
-// Replays an elimination order (EliminationTree::Eliminate +
-// Decompose, elimination_tree.cpp:24-70, elimination_tree.hpp:193-215):
-// the parent of v is the vertex whose elimination created v's component.
-int OrderToParent(int n, const uint64_t* adj, const int32_t* order,
-                  int8_t* parent) {
-  struct Comp {
-    uint64_t mask;
-    int creator, depth;
-  };
-  std::vector<Comp> comps;
-  auto split = [&](uint64_t set, int creator, int depth) {
-    while (set) {
-      uint64_t c = set & (0 - set), f = c;
-      while (f) {
-        int u = __builtin_ctzll(f);
-        f &= f - 1;
-        uint64_t nb = adj[u] & set & ~c;
-        c |= nb;
-        f |= nb;
-      }
-      comps.push_back({c, creator, depth});
-      set &= ~c;
-    }
-  };
-  split(n >= 64 ? ~0ull : ((1ull << n) - 1), -1, 0);
-  int td = 0;
-  for (int i = 0; i < n; ++i) {
-    const int v = order[i];
-    TD_REQUIRE(v >= 0 && v < n, TD_E_HIP, "elimination order out of range");
-    size_t k = 0;
-    while (k < comps.size() && !(comps[k].mask >> v & 1)) ++k;
-    TD_REQUIRE(k < comps.size(), TD_E_HIP, "vertex eliminated twice");
-    Comp c = comps[k];
-    comps.erase(comps.begin() + k);
-    parent[v] = static_cast<int8_t>(c.creator);
-    td = std::max(td, c.depth + 1);
-    split(c.mask & ~(1ull << v), v, c.depth + 1);
-  }
-  return td;
-}
-
 template <typename F>
 int Guard(td_ctx* ctx, F&& f) {
   if (!ctx) return TD_E_INVALID;
@@ -789,32 +803,38 @@ void SolveOne(td_ctx* ctx, int n, const uint64_t* adj, uint8_t* td_out,
   DPGraph G = MakeGraph(n, adj, lb, 0, cbits);
   const uint64_t tbytes = TableBytes(G);
   uint8_t* arena = static_cast<uint8_t*>(ctx->Buffer("dp.arena", tbytes));
-  DPGraph* dG = static_cast<DPGraph*>(ctx->Buffer("dp.graph", sizeof(DPGraph)));
+  // descriptor, walk start set and step count in one device block, uploaded
+  // from page-locked staging in one copy; the walk's order comes back the
+  // same way
+  constexpr size_t kStartOff = (sizeof(DPGraph) + 15) / 16 * 16;
+  constexpr size_t kInBytes = kStartOff + 16;
+  uint8_t* din = static_cast<uint8_t*>(ctx->Buffer("dp.graph", kInBytes));
+  DPGraph* dG = reinterpret_cast<DPGraph*>(din);
+  uint32_t* dstart = reinterpret_cast<uint32_t*>(din + kStartOff);
   const uint32_t* cb = PascalDevice(ctx);
-  uint32_t* dstart = static_cast<uint32_t*>(ctx->Buffer("dp.walk_start", 8));
-  int32_t* dsteps = static_cast<int32_t*>(ctx->Buffer("dp.walk_steps", 8));
-  int32_t* dorder = static_cast<int32_t*>(ctx->Buffer("dp.walk_order", 64 * 4));
+  // tree output: 64 parents then the depth
+  uint8_t* dtree = static_cast<uint8_t*>(ctx->Buffer("dp.tree_out", 64 + 16));
+  uint8_t* stage = static_cast<uint8_t*>(ctx->Pinned(kInBytes + 64 + 16));
   const uint32_t full = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
-  const int32_t steps = n;
-  TD_HIP(hipMemcpyAsync(dG, &G, sizeof(G), hipMemcpyHostToDevice, ctx->stream));
-  TD_HIP(hipMemcpyAsync(dstart, &full, 4, hipMemcpyHostToDevice, ctx->stream));
-  TD_HIP(hipMemcpyAsync(dsteps, &steps, 4, hipMemcpyHostToDevice, ctx->stream));
+  std::memcpy(stage, &G, sizeof(G));
+  std::memcpy(stage + kStartOff, &full, 4);
+  TD_HIP(hipMemcpyAsync(din, stage, kInBytes, hipMemcpyHostToDevice, ctx->stream));
   uint64_t launches = 0;
   TD_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
   RunLevels(ctx, G, dG, cb, arena, &launches);
   TD_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
-  hipLaunchKernelGGL(DPWalkKernel, dim3(1), dim3(64), 0, ctx->stream, dG, cb,
-                     arena, dstart, dsteps, dorder, 64);
+  hipLaunchKernelGGL(DPTreeKernel, dim3(1), dim3(64), 0, ctx->stream, dG, cb, arena, dstart,
+                     reinterpret_cast<int8_t*>(dtree), reinterpret_cast<int32_t*>(dtree + 64));
   TD_HIP(hipGetLastError());
   ++launches;
   TD_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-  int32_t order[64];
-  TD_HIP(hipMemcpyAsync(order, dorder, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  uint8_t* staged = stage + kInBytes;
+  TD_HIP(hipMemcpyAsync(staged, dtree, 64 + 4, hipMemcpyDeviceToHost, ctx->stream));
   TD_HIP(hipStreamSynchronize(ctx->stream));
-  int8_t parent[64];
-  const int td = OrderToParent(n, adj, order, parent);
+  int32_t td = 0;
+  std::memcpy(&td, staged + 64, 4);
   if (td_out) *td_out = static_cast<uint8_t>(td);
-  if (parent_out) std::memcpy(parent_out, parent, n);
+  if (parent_out) std::memcpy(parent_out, staged, n);
   ctx->last_n = n;
   ctx->last_graph = G;
   ctx->have_dp = true;
@@ -957,18 +977,14 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
   const uint32_t* cb = PascalDevice(ctx);
   uint32_t* dq = static_cast<uint32_t*>(ctx->Buffer("dp.batch_queue", 256));
   uint32_t* dstart = static_cast<uint32_t*>(ctx->Buffer("dp.batch_start", 4 * ngraphs));
-  int32_t* dsteps = static_cast<int32_t*>(ctx->Buffer("dp.batch_steps", 4 * ngraphs));
-  int32_t* dorder = static_cast<int32_t*>(ctx->Buffer("dp.batch_order", 4 * 32 * ngraphs));
+  int8_t* dpar = static_cast<int8_t*>(ctx->Buffer("dp.batch_parent", 64ull * ngraphs));
+  int32_t* dtd = static_cast<int32_t*>(ctx->Buffer("dp.batch_td", 4ull * ngraphs));
   std::vector<uint32_t> start(ngraphs);
-  std::vector<int32_t> steps(ngraphs);
-  for (int j = 0; j < ngraphs; ++j) {
+  for (int j = 0; j < ngraphs; ++j)
     start[j] = Gs[j].n >= 32 ? 0xFFFFFFFFu : ((1u << Gs[j].n) - 1u);
-    steps[j] = static_cast<int32_t>(Gs[j].n);
-  }
   TD_HIP(hipMemcpyAsync(dGs, Gs.data(), sizeof(DPGraph) * ngraphs,
                         hipMemcpyHostToDevice, ctx->stream));
   TD_HIP(hipMemcpyAsync(dstart, start.data(), 4 * ngraphs, hipMemcpyHostToDevice, ctx->stream));
-  TD_HIP(hipMemcpyAsync(dsteps, steps.data(), 4 * ngraphs, hipMemcpyHostToDevice, ctx->stream));
   uint64_t launches = 0;
   TD_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
   switch (lb) {
@@ -979,24 +995,22 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
     default: throw Error{TD_E_INVALID, "low_bits must be in [2, 5]"};
   }
   TD_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
-  hipLaunchKernelGGL(DPWalkKernel, dim3(ngraphs), dim3(64), 0, ctx->stream,
-                     dGs, cb, arena, dstart, dsteps, dorder, 32);
+  hipLaunchKernelGGL(DPTreeKernel, dim3(ngraphs), dim3(64), 0, ctx->stream,
+                     dGs, cb, arena, dstart, dpar, dtd);
   TD_HIP(hipGetLastError());
   ++launches;
   TD_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-  std::vector<int32_t> order(static_cast<size_t>(32) * ngraphs);
-  TD_HIP(hipMemcpyAsync(order.data(), dorder, order.size() * 4,
-                        hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<int8_t> par(64ull * ngraphs);
+  std::vector<int32_t> tds(ngraphs);
+  TD_HIP(hipMemcpyAsync(par.data(), dpar, par.size(), hipMemcpyDeviceToHost, ctx->stream));
+  TD_HIP(hipMemcpyAsync(tds.data(), dtd, 4ull * ngraphs, hipMemcpyDeviceToHost, ctx->stream));
   TD_HIP(hipStreamSynchronize(ctx->stream));
   uint64_t subsets = 0, alg = 0;
   for (int j = 0; j < ngraphs; ++j) {
     const int i = perm[j];
     const int n = ns[i];
-    int8_t parent[32];
-    const int td = OrderToParent(n, adj_concat + row_off[i],
-                                 order.data() + 32 * j, parent);
-    if (td_out) td_out[i] = static_cast<uint8_t>(td);
-    if (parent_concat) std::memcpy(parent_concat + row_off[i], parent, n);
+    if (td_out) td_out[i] = static_cast<uint8_t>(tds[j]);
+    if (parent_concat) std::memcpy(parent_concat + row_off[i], par.data() + 64ull * j, n);
     subsets += 1ull << n;
     alg += (1ull << (n - 1)) * static_cast<uint64_t>(n + 2);
   }

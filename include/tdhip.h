@@ -66,7 +66,7 @@ typedef struct td_dp_stats {
   uint64_t alg_bytes;    /* sum over graphs of 2^(n-1) (n+2)              */
   uint64_t launches;     /* kernel launches (or graph replays)            */
   double level_ms;       /* GPU time of the level kernels (HIP events)    */
-  double tree_ms;        /* GPU time of the elimination-order kernel      */
+  double tree_ms;        /* GPU time of the decomposition (tree) kernel   */
   double total_ms;       /* host wall time of the whole call              */
 } td_dp_stats;
 
