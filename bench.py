@@ -7,10 +7,13 @@ step = the full subset DP (all 2^30 subsets) plus the elimination-order walk
 and tree replay for each of the 8 graphs.  value = subsets resolved per second
 over the timed steps (wall clock; every call is synchronous).
 
-roofline: the level kernel (DPLevelKernel) against HBM peak, with
-achieved = algorithmic bytes / GPU time of the level launches (HIP events on
-the library's stream), algorithmic bytes = 2^(n-1) (n+2) per graph (k one-byte
-reads of level k-1 and one write per k-subset).
+roofline: the level kernels against HBM peak, with achieved = algorithmic
+bytes / GPU time of the level launches (HIP events on the library's stream
+around each graph's level sequence), algorithmic bytes = 2^(n-1) (n+2) per
+graph (k one-byte reads of level k-1 and one write per k-subset).  A graph's
+levels run as DPLevelKernel launches, except the runs of small levels at
+either end, which one DPSmallLevelsKernel workgroup does per run; per-launch
+figures divide by that launch count.
 
 Launch shape: td_dp_autotune picks (low_bits, tiles_per_lane, block) for n
 before the warm-up, unless --no-autotune or an explicit shape flag is given.
@@ -358,6 +361,8 @@ def main():
                   f"block={ctx.get_param('dp.block')}", file=sys.stderr, flush=True)
         dp = api.DynamicGPU(ctx)
 
+        kernels = [0]
+
         def step():
             lvl_ms = 0.0
             alg = 0
@@ -367,11 +372,13 @@ def main():
                 lvl_ms += dp.stats.level_ms
                 alg += dp.stats.alg_bytes
                 sub += dp.stats.subsets
+                kernels[0] += dp.stats.level_kernels
             return lvl_ms, alg, sub
 
         for _ in range(args.warmup):
             step()
         print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+        kernels[0] = 0
         t0 = time.perf_counter()
         lvl_ms = 0.0
         alg = sub = 0
@@ -383,7 +390,7 @@ def main():
             sub += c
             print(f"[bench] step {i} level_ms={a:.3f}", file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
-        nlev = sum(max(g.n - ctx.get_param("dp.low_bits"), 0) + 1 for g in graphs)
+        nlev = kernels[0]  # level-kernel executions in the timed steps
         achieved = alg / (lvl_ms * 1e-3) / 1e9
         n = graphs[0].n
         out = {
@@ -420,8 +427,9 @@ def main():
                 "frac": achieved / HBM_PEAK_GBPS,
                 "traffic": profile_traffic(),
                 "kernel": "DPLevelKernel",
-                "alg_bytes_per_launch": alg / (args.steps * nlev),
-                "avg_launch_ms": lvl_ms / (args.steps * nlev),
+                "alg_bytes_per_launch": alg / nlev,
+                "avg_launch_ms": lvl_ms / nlev,
+                "launches": nlev,
             },
             "extra": {
                 "graphs_per_s": len(graphs) * args.steps / wall,

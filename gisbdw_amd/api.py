@@ -47,6 +47,7 @@ class DPStats(C.Structure):
         ("level_ms", C.c_double),
         ("tree_ms", C.c_double),
         ("total_ms", C.c_double),
+        ("level_kernels", C.c_uint64),
     ]
 
     def as_dict(self):

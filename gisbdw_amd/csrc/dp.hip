@@ -401,11 +401,30 @@ __global__ __launch_bounds__(1024) void DPPersistentKernel(
     __syncthreads();
     const int nh = static_cast<int>(g->nh);
     for (int h = 0; h <= nh; ++h) {
-      const uint32_t ntiles = g->lvl[h + 1] - g->lvl[h];
+      const uint32_t ntiles = s_cb[nh * kCbStride + h];  // C(nh, h), no padding
       DPLevelRange<LB>(g, arena, h, 0, ntiles, threadIdx.x, blockDim.x * R, R,
                        s_cb, s_nbh);
       __syncthreads();
     }
+  }
+}
+
+// A run of small high levels [h0, h1] of one graph in one workgroup, with a
+// workgroup barrier between levels (the first and last levels of a large
+// graph have few tiles; one launch replaces h1 - h0 + 1 of them).
+template <int LB>
+__global__ __launch_bounds__(1024) void DPSmallLevelsKernel(
+    const DPGraph* __restrict__ g, const uint32_t* __restrict__ g_cb,
+    uint8_t* __restrict__ arena, int h0, int h1) {
+  __shared__ uint32_t s_cb[32 * kCbStride];
+  __shared__ uint32_t s_nbh[512];
+  LoadShared<LB>(g, g_cb, s_cb, s_nbh, true);
+  __syncthreads();
+  const int nh = static_cast<int>(g->nh);
+  for (int h = h0; h <= h1; ++h) {
+    const uint32_t ntiles = s_cb[nh * kCbStride + h];
+    DPLevelRange<LB>(g, arena, h, 0, ntiles, threadIdx.x, blockDim.x, 1, s_cb, s_nbh);
+    __syncthreads();
   }
 }
 
@@ -686,6 +705,26 @@ ClassSched MakeClassSched(const DPGraph& G, int h) {
   return S;
 }
 
+// Runs of small levels at either end of a graph's level sequence go to one
+// workgroup each (DPSmallLevelsKernel): levels [0, lo] and [hi, nh].
+constexpr uint64_t kSmallTiles = 320;
+void SmallLevelRuns(int nh, int* lo, int* hi) {
+  int a = -1, b = nh + 1;
+  while (a + 1 <= nh && Binom(nh, a + 1) <= kSmallTiles) ++a;
+  while (b - 1 > a && Binom(nh, b - 1) <= kSmallTiles) --b;
+  *lo = a;
+  *hi = b;
+}
+
+// level kernels one solve of G executes
+uint64_t LevelKernelCount(const DPGraph& G) {
+  const int nh = static_cast<int>(G.nh);
+  if (G.cb > 0) return static_cast<uint64_t>(nh + 1);
+  int lo, hi;
+  SmallLevelRuns(nh, &lo, &hi);
+  return static_cast<uint64_t>((lo >= 0 ? 1 : 0) + (hi - lo - 1) + (hi <= nh ? 1 : 0));
+}
+
 template <int LB>
 void LaunchLevels(td_ctx* ctx, const DPGraph& G, const DPGraph* dG,
                   const uint32_t* cb, uint8_t* arena, uint64_t* launches) {
@@ -703,12 +742,25 @@ void LaunchLevels(td_ctx* ctx, const DPGraph& G, const DPGraph* dG,
     TD_HIP(hipGetLastError());
     return;
   }
-  for (int h = 0; h <= static_cast<int>(G.nh); ++h) {
+  const int nh = static_cast<int>(G.nh);
+  int lo, hi;
+  SmallLevelRuns(nh, &lo, &hi);
+  if (lo >= 0) {
+    hipLaunchKernelGGL(DPSmallLevelsKernel<LB>, dim3(1), dim3(1024), 0, ctx->stream, dG, cb,
+                       arena, 0, lo);
+    ++*launches;
+  }
+  for (int h = lo + 1; h < hi; ++h) {
     const uint32_t ntiles = static_cast<uint32_t>(Binom(G.nh, h));
     int grid = GridFor(ctx, (ntiles + R - 1) / R, block);
     if (ctx->dp.xcd_remap && grid >= 64) grid = (grid + 7) / 8 * 8;
     hipLaunchKernelGGL(DPLevelKernel<LB>, dim3(grid), dim3(block), 0,
                        ctx->stream, dG, cb, arena, h, ntiles, R, ctx->dp.xcd_remap);
+    ++*launches;
+  }
+  if (hi <= nh) {
+    hipLaunchKernelGGL(DPSmallLevelsKernel<LB>, dim3(1), dim3(1024), 0, ctx->stream, dG, cb,
+                       arena, hi, nh);
     ++*launches;
   }
   TD_HIP(hipGetLastError());
@@ -840,6 +892,7 @@ void SolveOne(td_ctx* ctx, int n, const uint64_t* adj, uint8_t* td_out,
   ctx->have_dp = true;
   if (stats) {
     stats->graphs = 1;
+    stats->level_kernels = LevelKernelCount(G);
     stats->subsets = 1ull << n;
     stats->alg_bytes = (1ull << (n - 1)) * static_cast<uint64_t>(n + 2);
     stats->launches = launches;
@@ -995,6 +1048,8 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
     default: throw Error{TD_E_INVALID, "low_bits must be in [2, 5]"};
   }
   TD_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+  // mode 0: the persistent kernel (its queue reset aside); else the launches
+  const uint64_t level_kernels = mode == 0 ? 1 : launches;
   hipLaunchKernelGGL(DPTreeKernel, dim3(ngraphs), dim3(64), 0, ctx->stream,
                      dGs, cb, arena, dstart, dpar, dtd);
   TD_HIP(hipGetLastError());
@@ -1017,6 +1072,7 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
   ctx->have_dp = false;  // queries refer to single-graph solves only
   if (stats) {
     stats->graphs = static_cast<uint64_t>(ngraphs);
+    stats->level_kernels = level_kernels;
     stats->subsets = subsets;
     stats->alg_bytes = alg;
     stats->launches = launches;

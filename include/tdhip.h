@@ -68,6 +68,7 @@ typedef struct td_dp_stats {
   double level_ms;       /* GPU time of the level kernels (HIP events)    */
   double tree_ms;        /* GPU time of the decomposition (tree) kernel   */
   double total_ms;       /* host wall time of the whole call              */
+  uint64_t level_kernels; /* level-kernel executions behind level_ms       */
 } td_dp_stats;
 
 /* DynamicGPU::operator()(g) + GetElimination(n, n, 0) + EliminationTree +
