@@ -524,7 +524,8 @@ __global__ __launch_bounds__(64) void DPTreeKernel(
   const uint32_t myadj = lane < 32 ? g->adj[lane] : 0u;
   uint32_t S = start[blockIdx.x];
   int creator = -1, depth = 0, parent = -1;
-  while (S) {
+  // every round removes at least one vertex per component, so at most 32
+  for (int round = 0; round < 32 && S; ++round) {
     const bool in = lane < 32 && ((S >> lane) & 1u);
     uint32_t rest = S, mine = 0u;
     while (rest) {
@@ -542,8 +543,10 @@ __global__ __launch_bounds__(64) void DPTreeKernel(
     const uint32_t tdc = in ? Lookup(g, arena, s_cb, mine) : 0u;
     const uint32_t tv = in ? Lookup(g, arena, s_cb, mine & ~(1u << lane)) : 0xFFu;
     const uint32_t cand = static_cast<uint32_t>(__ballot(in && tv + 1u == tdc));
+    // a connected set always has a vertex with td(C \ v) = td(C) - 1; the
+    // lowest vertex stands in only so that a damaged table cannot stall
     const uint32_t rc = cand & mine;
-    const int root = in ? __builtin_ctz(rc) : 0;  // every component has a candidate
+    const int root = in ? __builtin_ctz(rc ? rc : mine) : 0;
     const int cdepth = __shfl(depth, creator >= 0 ? creator : lane);
     if (in && root == lane) {
       parent = creator;
