@@ -83,8 +83,8 @@ def bnb_extra(ctx, budget=50000):
     from gisbdw_amd import graphs as G
 
     ctx.set_param("bnb.node_budget", budget)
-    ctx.set_param("bnb.waves_per_graph", 4)
-    out = {"node_budget_per_graph": budget, "waves_per_graph": 4}
+    ctx.set_param("bnb.waves_per_graph", 8)
+    out = {"node_budget_per_graph": budget, "waves_per_graph": 8}
 
     def families(W, cert):
         fam = {}

@@ -45,9 +45,11 @@ const char* td_last_error(const td_ctx* ctx);
  * "dp.batch_wg_per_cu", "dp.use_graph" (0/1), "dp.class_bits" (0, 3 or 4,
  * default 0: level tables grouped into 2^bits vertex classes, one class per
  * XCD, for single graphs with at least "dp.class_min_nh" (default 16)
- * vertices above the low bits), "bnb.waves", "bnb.node_budget"
- * (per graph, default 2e6), "bnb.waves_per_graph" (1..4, default 4),
- * "bnb.groups" (workgroups in flight), "bnb.memo_log2" (default 17). */
+ * vertices above the low bits), "bnb.node_budget"
+ * (per graph, default 2e6), "bnb.waves_per_graph" (1..8, default 8),
+ * "bnb.groups" (workgroups in flight), "bnb.memo_log2" (memo buckets per
+ * wave, 4 entries of 16 bytes each; default sized to the per-wave node
+ * budget within [2^10, 2^17]). */
 int td_ctx_set_param(td_ctx* ctx, const char* key, int64_t value);
 int td_ctx_get_param(const td_ctx* ctx, const char* key, int64_t* value);
 

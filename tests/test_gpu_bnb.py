@@ -109,7 +109,7 @@ def test_bnb_dense_exact_when_certified(ctx, oracle):
             assert t == w, g.name
 
 
-@pytest.mark.parametrize("waves", [1, 2, 4])
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
 def test_bnb_waves_per_graph_same_answers(ctx, oracle, waves):
     rng = random.Random(90 + waves)
     gs = [G.gnp(rng.randint(8, 22), rng.uniform(0.1, 0.35), rng.getrandbits(32)) for _ in range(20)]
@@ -118,7 +118,7 @@ def test_bnb_waves_per_graph_same_answers(ctx, oracle, waves):
     try:
         tds, pars, cert, st = api.bnb_solve_batch(ctx, gs)
     finally:
-        ctx.set_param("bnb.waves_per_graph", 4)
+        ctx.set_param("bnb.waves_per_graph", 8)
     heur, lb, nodes = api.bnb_last_details(ctx, len(gs))
     for i, (g, t, p, c) in enumerate(zip(gs, tds, pars, cert)):
         assert oracle.certify(g, p, t)
