@@ -38,7 +38,13 @@ constexpr int P_W0 = 0, P_B0 = 256, P_WS1 = 320, P_WN1 = 4416, P_B1 = 8512,
 // per-graph working set, in LDS: 7 activation matrices and, for training,
 // one bf16 gradient matrix (64x64 each) and two fp32 matrices: 56 KiB for
 // scoring, 96 KiB for training (gfx950 has 160 KiB of LDS per CU)
-constexpr int kMat = kH * kH;
+// LDS row strides: 72 bf16 (144 bytes: rows stay 16-byte aligned for the
+// MFMA operand loads, and lane v walking row v meets at most 4-way bank
+// conflicts) and 65 fp32 (conflict-free)
+constexpr int kLd = kH + 8;
+constexpr int kLdF = kH + 1;
+constexpr int kMatLd = kH * kLd;
+constexpr int kMatLdF = kH * kLdF;
 enum { A_H0 = 0, A_M1, A_H1, A_M2, A_H2, A_Z1, A_Z2, G_A, kNBf16 };
 constexpr int kScoreMats = A_Z2 + 1;
 
@@ -72,7 +78,10 @@ __device__ __forceinline__ void Zero(Acc& a) {
 
 // acc += A (64xK=64) * B (64x64); A(m,k) = A[m*am + k*ak], B(k,n) =
 // B[k*bk + n*bn], bf16 bit patterns.  Lane l supplies row/col l%16 and the
-// k-group l/16 (8 consecutive k) of each 16x16x32 step.
+// k-group l/16 (8 consecutive k) of each 16x16x32 step.  kAc / kBc: ak == 1
+// / bk == 1, so a lane's 8 k-values are 16 contiguous, 16-byte aligned
+// bytes and come in one load.
+template <bool kAc, bool kBc>
 __device__ __forceinline__ void MatMul(Acc& acc, const uint16_t* __restrict__ A,
                                        int am, int ak,
                                        const uint16_t* __restrict__ B, int bk,
@@ -82,13 +91,22 @@ __device__ __forceinline__ void MatMul(Acc& acc, const uint16_t* __restrict__ A,
 #pragma unroll
   for (int ks = 0; ks < 2; ++ks) {
     v8s a[4], b[4];
+    const int k0 = 32 * ks + 8 * g;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
+      if constexpr (kAc) {
+        a[t] = __builtin_bit_cast(v8s, *reinterpret_cast<const uint4*>(A + (16 * t + r) * am + k0));
+      } else {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int k = 32 * ks + 8 * g + j;
-        a[t][j] = static_cast<short>(A[(16 * t + r) * am + k * ak]);
-        b[t][j] = static_cast<short>(B[k * bk + (16 * t + r) * bn]);
+        for (int j = 0; j < 8; ++j)
+          a[t][j] = static_cast<short>(A[(16 * t + r) * am + (k0 + j) * ak]);
+      }
+      if constexpr (kBc) {
+        b[t] = __builtin_bit_cast(v8s, *reinterpret_cast<const uint4*>(B + (16 * t + r) * bn + k0));
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          b[t][j] = static_cast<short>(B[(k0 + j) * bk + (16 * t + r) * bn]);
       }
     }
 #pragma unroll
@@ -118,7 +136,7 @@ __device__ __forceinline__ void StoreReluBf16(const Acc& acc, const float* bias,
       for (int i = 0; i < 4; ++i) {
         const int row = AccRow(tm, i), col = AccCol(tn);
         const float v = acc.t[tm][tn][i] + bias[col];
-        Y[row * kH + col] = ToBf16(v > 0.f ? v : 0.f);
+        Y[row * kLd + col] = ToBf16(v > 0.f ? v : 0.f);
       }
 }
 
@@ -140,15 +158,26 @@ __device__ __forceinline__ void Aggregate(const uint16_t* h, uint16_t* m,
   const float inv = deg ? 1.0f / static_cast<float>(deg) : 0.f;
   for (int c = 0; c < kH; ++c) {
     float s = 0.f;
-    for (uint64_t a = myadj; a; a &= a - 1) s += FromBf16(h[__builtin_ctzll(a) * kH + c]);
-    m[v * kH + c] = ToBf16(s * inv);
+    for (uint64_t a = myadj; a; a &= a - 1) s += FromBf16(h[__builtin_ctzll(a) * kLd + c]);
+    m[v * kLd + c] = ToBf16(s * inv);
   }
 }
 
 struct Net {
   const float* w;      // fp32 master (biases, W0 and w5 are read as bf16 values)
   const uint16_t* wb;  // bf16 copy
+  const uint16_t* wt;  // bf16 copy with each 64x64 matrix transposed
 };
+
+// index of parameter p in the copy with the 64x64 matrices transposed
+__host__ __device__ inline int TransposedIndex(int p) {
+  const int bases[6] = {P_WS1, P_WN1, P_WS2, P_WN2, P_W3, P_W4};
+  for (int i = 0; i < 6; ++i) {
+    const int q = p - bases[i];
+    if (q >= 0 && q < kH * kH) return bases[i] + (q % kH) * kH + q / kH;
+  }
+  return p;
+}
 
 __device__ __forceinline__ float Wb(const Net& N, int p) { return FromBf16(N.wb[p]); }
 
@@ -157,46 +186,46 @@ __device__ __forceinline__ float Wb(const Net& N, int p) { return FromBf16(N.wb[
 __device__ float Forward(const Net& N, const float* __restrict__ x, uint64_t myadj,
                          uint16_t* S) {
   const int v = threadIdx.x & 63;
-  uint16_t* h0 = S + A_H0 * kMat;
-  uint16_t* m1 = S + A_M1 * kMat;
-  uint16_t* h1 = S + A_H1 * kMat;
-  uint16_t* m2 = S + A_M2 * kMat;
-  uint16_t* h2 = S + A_H2 * kMat;
-  uint16_t* z1 = S + A_Z1 * kMat;
-  uint16_t* z2 = S + A_Z2 * kMat;
+  uint16_t* h0 = S + A_H0 * kMatLd;
+  uint16_t* m1 = S + A_M1 * kMatLd;
+  uint16_t* h1 = S + A_H1 * kMatLd;
+  uint16_t* m2 = S + A_M2 * kMatLd;
+  uint16_t* h2 = S + A_H2 * kMatLd;
+  uint16_t* z1 = S + A_Z1 * kMatLd;
+  uint16_t* z2 = S + A_Z2 * kMatLd;
   const float xv[4] = {x[v * 4 + 0], x[v * 4 + 1], x[v * 4 + 2], x[v * 4 + 3]};
   for (int o = 0; o < kH; ++o) {
     float a = N.w[P_B0 + o];
 #pragma unroll
     for (int i = 0; i < 4; ++i) a += Wb(N, P_W0 + o * 4 + i) * xv[i];
-    h0[v * kH + o] = ToBf16(a > 0.f ? a : 0.f);
+    h0[v * kLd + o] = ToBf16(a > 0.f ? a : 0.f);
   }
   Sync();
   Aggregate(h0, m1, myadj);
   Sync();
   Acc acc;
   Zero(acc);
-  MatMul(acc, h0, kH, 1, N.wb + P_WS1, 1, kH);
-  MatMul(acc, m1, kH, 1, N.wb + P_WN1, 1, kH);
+  MatMul<true, true>(acc, h0, kLd, 1, N.wb + P_WS1, 1, kH);
+  MatMul<true, true>(acc, m1, kLd, 1, N.wb + P_WN1, 1, kH);
   StoreReluBf16(acc, N.w + P_B1, h1);
   Sync();
   Aggregate(h1, m2, myadj);
   Sync();
   Zero(acc);
-  MatMul(acc, h1, kH, 1, N.wb + P_WS2, 1, kH);
-  MatMul(acc, m2, kH, 1, N.wb + P_WN2, 1, kH);
+  MatMul<true, true>(acc, h1, kLd, 1, N.wb + P_WS2, 1, kH);
+  MatMul<true, true>(acc, m2, kLd, 1, N.wb + P_WN2, 1, kH);
   StoreReluBf16(acc, N.w + P_B2, h2);
   Sync();
   Zero(acc);
-  MatMul(acc, h2, kH, 1, N.wb + P_W3, 1, kH);
+  MatMul<true, true>(acc, h2, kLd, 1, N.wb + P_W3, 1, kH);
   StoreReluBf16(acc, N.w + P_B3, z1);
   Sync();
   Zero(acc);
-  MatMul(acc, z1, kH, 1, N.wb + P_W4, 1, kH);
+  MatMul<true, true>(acc, z1, kLd, 1, N.wb + P_W4, 1, kH);
   StoreReluBf16(acc, N.w + P_B4, z2);
   Sync();
   float s = N.w[P_B5];
-  for (int j = 0; j < kH; ++j) s += FromBf16(z2[v * kH + j]) * Wb(N, P_W5 + j);
+  for (int j = 0; j < kH; ++j) s += FromBf16(z2[v * kLd + j]) * Wb(N, P_W5 + j);
   return s;
 }
 
@@ -273,7 +302,7 @@ __global__ __launch_bounds__(64) void MLPScoreKernel(Net N, const int32_t* __res
                                                      const uint64_t* __restrict__ adj,
                                                      const float* __restrict__ feat,
                                                      float* __restrict__ scores) {
-  __shared__ uint16_t s_act[kScoreMats * kMat];
+  __shared__ uint16_t s_act[kScoreMats * kMatLd];
   const int gi = blockIdx.x, v = threadIdx.x;
   const int n = ns[gi];
   const uint64_t myadj = v < n ? adj[static_cast<size_t>(gi) * 64 + v] : 0ull;
@@ -294,8 +323,10 @@ __global__ __launch_bounds__(64) void MLPTrainKernel(
   const int n = ns[gi];
   const bool in = v < n;
   const uint64_t myadj = in ? adj[static_cast<size_t>(gi) * 64 + v] : 0ull;
-  __shared__ uint16_t s_act[kNBf16 * kMat];
-  __shared__ float s_f[2 * kMat];
+  __shared__ uint16_t s_act[kNBf16 * kMatLd];
+  __shared__ float s_f[2 * kMatLdF];
+  __shared__ float s_degf[64];
+  s_degf[v] = static_cast<float>(__builtin_popcountll(myadj));  // read after Forward's syncs
   uint16_t* S = s_act;
   float* SF = s_f;
   float* G = part + static_cast<size_t>(bi) * kNParams;
@@ -313,28 +344,28 @@ __global__ __launch_bounds__(64) void MLPTrainKernel(
   const float lsum = WaveSumF(li);
   if (v == 0) loss[bi] = lsum;
   const float ds = in ? (p - t) * scale : 0.f;  // dL/ds_v
-  uint16_t* h0 = S + A_H0 * kMat;
-  uint16_t* m1 = S + A_M1 * kMat;
-  uint16_t* h1 = S + A_H1 * kMat;
-  uint16_t* m2 = S + A_M2 * kMat;
-  uint16_t* h2 = S + A_H2 * kMat;
-  uint16_t* z1 = S + A_Z1 * kMat;
-  uint16_t* z2 = S + A_Z2 * kMat;
-  uint16_t* gA = S + G_A * kMat;  // bf16 gradient (pre-activation)
-  float* F0 = SF;                 // fp32 gradient w.r.t. an activation
-  float* F1 = SF + kMat;
+  uint16_t* h0 = S + A_H0 * kMatLd;
+  uint16_t* m1 = S + A_M1 * kMatLd;
+  uint16_t* h1 = S + A_H1 * kMatLd;
+  uint16_t* m2 = S + A_M2 * kMatLd;
+  uint16_t* h2 = S + A_H2 * kMatLd;
+  uint16_t* z1 = S + A_Z1 * kMatLd;
+  uint16_t* z2 = S + A_Z2 * kMatLd;
+  uint16_t* gA = S + G_A * kMatLd;  // bf16 gradient (pre-activation)
+  float* F0 = SF;                   // fp32 gradient w.r.t. an activation
+  float* F1 = SF + kMatLdF;
   // ---- head: s = w5 . z2 + b5
   // dw5[j] = sum_r ds_r z2[r][j] (lane j); db5 = sum_r ds_r
   {
     float dw = 0.f;
-    for (int r = 0; r < kH; ++r) dw += __shfl(ds, r) * FromBf16(z2[r * kH + v]);
+    for (int r = 0; r < kH; ++r) dw += __shfl(ds, r) * FromBf16(z2[r * kLd + v]);
     G[P_W5 + v] = dw;
     const float db = WaveSumF(ds);
     if (v == 0) G[P_B5] = db;
     // da4[r][j] = ds_r w5[j] (z2 > 0)  (lane r)
     for (int j = 0; j < kH; ++j) {
-      const float zz = FromBf16(z2[v * kH + j]);
-      gA[v * kH + j] = ToBf16(zz > 0.f ? ds * Wb(N, P_W5 + j) : 0.f);
+      const float zz = FromBf16(z2[v * kLd + j]);
+      gA[v * kLd + j] = ToBf16(zz > 0.f ? ds * Wb(N, P_W5 + j) : 0.f);
     }
   }
   Sync();
@@ -344,31 +375,31 @@ __global__ __launch_bounds__(64) void MLPTrainKernel(
   auto dense_back = [&](const uint16_t* Xin, int pW, int pB, float* Fout) {
     Acc acc;
     Zero(acc);
-    MatMul(acc, gA, 1, kH, Xin, kH, 1);  // (o, r) x (r, i)
+    MatMul<false, false>(acc, gA, 1, kLd, Xin, kLd, 1);  // (o, r) x (r, i)
     StoreF32(acc, G + pW, kH, 1);
     float db = 0.f;
-    for (int r = 0; r < kH; ++r) db += FromBf16(gA[r * kH + v]);
+    for (int r = 0; r < kH; ++r) db += FromBf16(gA[r * kLd + v]);
     G[pB + v] = db;
     if (Fout) {
       Zero(acc);
-      MatMul(acc, gA, kH, 1, N.wb + pW, kH, 1);  // (r, o) x (o, i)
-      StoreF32(acc, Fout, kH, 1);
+      MatMul<true, true>(acc, gA, kLd, 1, N.wt + pW, 1, kH);  // (r, o) x (o, i)
+      StoreF32(acc, Fout, kLdF, 1);
     }
   };
   // layer 4: z2 = relu(W4 z1 + b4)
   dense_back(z1, P_W4, P_B4, F0);
   Sync();
   for (int j = 0; j < kH; ++j) {
-    const float y = FromBf16(z1[v * kH + j]);
-    gA[v * kH + j] = ToBf16(y > 0.f ? F0[v * kH + j] : 0.f);
+    const float y = FromBf16(z1[v * kLd + j]);
+    gA[v * kLd + j] = ToBf16(y > 0.f ? F0[v * kLdF + j] : 0.f);
   }
   Sync();
   // layer 3: z1 = relu(W3 h2 + b3)
   dense_back(h2, P_W3, P_B3, F0);
   Sync();
   for (int j = 0; j < kH; ++j) {
-    const float y = FromBf16(h2[v * kH + j]);
-    gA[v * kH + j] = ToBf16(y > 0.f ? F0[v * kH + j] : 0.f);
+    const float y = FromBf16(h2[v * kLd + j]);
+    gA[v * kLd + j] = ToBf16(y > 0.f ? F0[v * kLdF + j] : 0.f);
   }
   Sync();
   // message-passing layer: h_out = relu(Ws h_in + Wn m + b), m = mean_N h_in
@@ -376,28 +407,27 @@ __global__ __launch_bounds__(64) void MLPTrainKernel(
     dense_back(hin, pWs, pB, F0);  // dWs, db, F0 = gA Ws
     Acc acc;
     Zero(acc);
-    MatMul(acc, gA, 1, kH, mag, kH, 1);
+    MatMul<false, false>(acc, gA, 1, kLd, mag, kLd, 1);
     StoreF32(acc, G + pWn, kH, 1);
     Zero(acc);
-    MatMul(acc, gA, kH, 1, N.wb + pWn, kH, 1);
-    StoreF32(acc, F1, kH, 1);  // dL/dm
+    MatMul<true, true>(acc, gA, kLd, 1, N.wt + pWn, 1, kH);
+    StoreF32(acc, F1, kLdF, 1);  // dL/dm
     Sync();
-    // d h_in[u] += sum_{v in N(u)} dm[v] / deg_v   (lane u)
+    // d h_in[u] += sum_{v in N(u)} dm[v] / deg_v   (lane u; degrees from LDS)
     for (int c = 0; c < kH; ++c) {
-      float a = F0[v * kH + c];
+      float a = F0[v * kLdF + c];
       for (uint64_t nb = myadj; nb; nb &= nb - 1) {
         const int w = __builtin_ctzll(nb);
-        const int dw = __builtin_popcountll(adj[static_cast<size_t>(gi) * 64 + w]);
-        a += F1[w * kH + c] / static_cast<float>(dw);
+        a += F1[w * kLdF + c] / s_degf[w];
       }
-      F0[v * kH + c] = a;
+      F0[v * kLdF + c] = a;
     }
     Sync();
   };
   mp_back(h1, m2, P_WS2, P_WN2, P_B2);
   for (int j = 0; j < kH; ++j) {
-    const float y = FromBf16(h1[v * kH + j]);
-    gA[v * kH + j] = ToBf16(y > 0.f ? F0[v * kH + j] : 0.f);
+    const float y = FromBf16(h1[v * kLd + j]);
+    gA[v * kLd + j] = ToBf16(y > 0.f ? F0[v * kLdF + j] : 0.f);
   }
   Sync();
   mp_back(h0, m1, P_WS1, P_WN1, P_B1);
@@ -406,8 +436,8 @@ __global__ __launch_bounds__(64) void MLPTrainKernel(
     // lane o accumulates dW0[o][0..3] and db0[o] over rows r
     float dw[4] = {0.f, 0.f, 0.f, 0.f}, db = 0.f;
     for (int r = 0; r < kH; ++r) {
-      const float y = FromBf16(h0[r * kH + v]);
-      const float da = y > 0.f ? F0[r * kH + v] : 0.f;
+      const float y = FromBf16(h0[r * kLd + v]);
+      const float da = y > 0.f ? F0[r * kLdF + v] : 0.f;
       db += da;
 #pragma unroll
       for (int i = 0; i < 4; ++i) dw[i] += da * x[r * 4 + i];
@@ -451,6 +481,7 @@ __global__ __launch_bounds__(256) void MLPReduceKernel(const float* __restrict__
 }
 
 __global__ void MLPAdamKernel(float* __restrict__ w, uint16_t* __restrict__ wb,
+                              uint16_t* __restrict__ wt,
                               float* __restrict__ m1, float* __restrict__ m2,
                               const float* __restrict__ g, float lr, float b1,
                               float b2, float eps, float c1, float c2) {
@@ -464,11 +495,16 @@ __global__ void MLPAdamKernel(float* __restrict__ w, uint16_t* __restrict__ wb,
   const float nw = w[p] - lr * (a / c1) / (sqrtf(b / c2) + eps);
   w[p] = nw;
   wb[p] = ToBf16(nw);
+  wt[TransposedIndex(p)] = ToBf16(nw);
 }
 
-__global__ void MLPToBf16Kernel(const float* __restrict__ w, uint16_t* __restrict__ wb) {
+__global__ void MLPToBf16Kernel(const float* __restrict__ w, uint16_t* __restrict__ wb,
+                                uint16_t* __restrict__ wt) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < kNParams) wb[p] = ToBf16(w[p]);
+  if (p < kNParams) {
+    wb[p] = ToBf16(w[p]);
+    wt[TransposedIndex(p)] = ToBf16(w[p]);
+  }
 }
 
 // Optimal-root labels from the last single-graph DP: for a connected vertex
@@ -530,6 +566,7 @@ int Guard(td_ctx* ctx, F&& f) {
 struct DevNet {
   float* w;
   uint16_t* wb;
+  uint16_t* wt;
   float* m1;
   float* m2;
 };
@@ -539,11 +576,13 @@ DevNet NetBuffers(td_ctx* ctx) {
   const bool fresh = ctx->bufs.find("mlp.w") == ctx->bufs.end();
   d.w = static_cast<float*>(ctx->Buffer("mlp.w", kNParams * 4));
   d.wb = static_cast<uint16_t*>(ctx->Buffer("mlp.wb", kNParams * 2));
+  d.wt = static_cast<uint16_t*>(ctx->Buffer("mlp.wt", kNParams * 2));
   d.m1 = static_cast<float*>(ctx->Buffer("mlp.adam1", kNParams * 4));
   d.m2 = static_cast<float*>(ctx->Buffer("mlp.adam2", kNParams * 4));
   if (fresh) {
     TD_HIP(hipMemsetAsync(d.w, 0, kNParams * 4, ctx->stream));
     TD_HIP(hipMemsetAsync(d.wb, 0, kNParams * 2, ctx->stream));
+    TD_HIP(hipMemsetAsync(d.wt, 0, kNParams * 2, ctx->stream));
     TD_HIP(hipMemsetAsync(d.m1, 0, kNParams * 4, ctx->stream));
     TD_HIP(hipMemsetAsync(d.m2, 0, kNParams * 4, ctx->stream));
   }
@@ -613,7 +652,7 @@ int td_mlp_init(td_ctx* ctx, uint64_t seed) {
     TD_HIP(hipMemsetAsync(d.m1, 0, kNParams * 4, ctx->stream));
     TD_HIP(hipMemsetAsync(d.m2, 0, kNParams * 4, ctx->stream));
     hipLaunchKernelGGL(MLPToBf16Kernel, dim3((kNParams + 255) / 256), dim3(256), 0,
-                       ctx->stream, d.w, d.wb);
+                       ctx->stream, d.w, d.wb, d.wt);
     TD_HIP(hipGetLastError());
     ctx->params["mlp.adam_t"] = 0;
     TD_HIP(hipStreamSynchronize(ctx->stream));
@@ -628,7 +667,7 @@ int td_mlp_set_weights(td_ctx* ctx, const float* w, int64_t count) {
     TD_HIP(hipMemsetAsync(d.m1, 0, kNParams * 4, ctx->stream));
     TD_HIP(hipMemsetAsync(d.m2, 0, kNParams * 4, ctx->stream));
     hipLaunchKernelGGL(MLPToBf16Kernel, dim3((kNParams + 255) / 256), dim3(256), 0,
-                       ctx->stream, d.w, d.wb);
+                       ctx->stream, d.w, d.wb, d.wt);
     TD_HIP(hipGetLastError());
     ctx->params["mlp.adam_t"] = 0;
     TD_HIP(hipStreamSynchronize(ctx->stream));
@@ -663,7 +702,7 @@ int td_mlp_scores(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint64_
     Uploaded u = UploadGraphs(ctx, "mlp.in", ngraphs, ns, adj64);
     auto* sc = static_cast<float*>(ctx->Buffer("mlp.scores", 256ull * ngraphs));
     hipLaunchKernelGGL(MLPScoreKernel, dim3(ngraphs), dim3(64), 0, ctx->stream,
-                       Net{d.w, d.wb}, u.ns, u.adj, u.feat, sc);
+                       Net{d.w, d.wb, d.wt}, u.ns, u.adj, u.feat, sc);
     TD_HIP(hipGetLastError());
     TD_HIP(hipMemcpyAsync(scores_out, sc, 256ull * ngraphs, hipMemcpyDeviceToHost,
                           ctx->stream));
@@ -691,7 +730,7 @@ int td_mlp_loss_grad(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint
     auto* loss = static_cast<float*>(ctx->Buffer("mlp.loss", 4ull * ngraphs));
     auto* grad = static_cast<float*>(ctx->Buffer("mlp.grad", 4ull * kNParams));
     hipLaunchKernelGGL(MLPTrainKernel, dim3(ngraphs), dim3(64), 0, ctx->stream,
-                       Net{d.w, d.wb}, u.ns, u.adj, tg, u.feat, nullptr, part, loss,
+                       Net{d.w, d.wb, d.wt}, u.ns, u.adj, tg, u.feat, nullptr, part, loss,
                        1.0f / static_cast<float>(ngraphs));
     hipLaunchKernelGGL(MLPReduceKernel, dim3((kNParams + kReduceParams - 1) / kReduceParams),
                        dim3(256), 0, ctx->stream, part, ngraphs, grad);
@@ -745,7 +784,7 @@ int td_mlp_train(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint64_t
       int nb = 0;
       for (int s0 = 0; s0 + batch <= ngraphs; s0 += batch, ++nb) {
         hipLaunchKernelGGL(MLPTrainKernel, dim3(batch), dim3(64), 0, ctx->stream,
-                           Net{d.w, d.wb}, u.ns, u.adj, tg, u.feat, idx + s0, part, loss + s0,
+                           Net{d.w, d.wb, d.wt}, u.ns, u.adj, tg, u.feat, idx + s0, part, loss + s0,
                            1.0f / static_cast<float>(batch));
         hipLaunchKernelGGL(MLPReduceKernel,
                            dim3((kNParams + kReduceParams - 1) / kReduceParams), dim3(256), 0,
@@ -754,7 +793,7 @@ int td_mlp_train(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint64_t
         const float c1 = 1.f - std::pow(b1, static_cast<float>(t));
         const float c2 = 1.f - std::pow(b2, static_cast<float>(t));
         hipLaunchKernelGGL(MLPAdamKernel, dim3((kNParams + 255) / 256), dim3(256), 0,
-                           ctx->stream, d.w, d.wb, d.m1, d.m2, grad, lr, b1, b2, eps, c1, c2);
+                           ctx->stream, d.w, d.wb, d.wt, d.m1, d.m2, grad, lr, b1, b2, eps, c1, c2);
       }
       TD_HIP(hipGetLastError());
       std::vector<float> l(static_cast<size_t>(nb) * batch);
