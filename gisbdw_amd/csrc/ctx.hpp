@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 #include <map>
 #include <string>
@@ -43,8 +44,20 @@ struct DPGraph {
   uint32_t lvl[34];  // tile offset of high level h; lvl[nh+1] = 2^nh
   uint32_t adj[32];  // adjacency rows
   uint32_t nl[32];   // N(L) for every L subset of the low vertices
-  uint32_t nbh[512]; // N(H) by 7-bit chunks of H: nbh[128 c + chunk]
+  // The next three arrays are consecutive (the level kernels copy them to
+  // LDS as one block).  Masks over the low parts L of a tile use the
+  // tile-mask bit order: bit 8 (L & 3) + (L >> 2) stands for L.
+  uint32_t nbh[512];  // N(H) by 7-bit chunks of H: nbh[128 c + chunk]
+  uint32_t isoh[512]; // by 7-bit chunks of high vertices X: the L that give
+                      // every x in X a neighbour
+  uint32_t isol[32];  // by set Y of low vertices: the L in which every
+                      // y in Y n L has a neighbour in L
 };
+constexpr int kTabNbh = 0, kTabIsoH = 512, kTabIsoL = 1024, kTabWords = 1056;
+static_assert(offsetof(DPGraph, isoh) == offsetof(DPGraph, nbh) + 4 * kTabIsoH,
+              "nbh and isoh must be consecutive");
+static_assert(offsetof(DPGraph, isol) == offsetof(DPGraph, nbh) + 4 * kTabIsoL,
+              "isoh and isol must be consecutive");
 
 // Class-major level layout.  High level h (|H| = h) starts at tile lvl[h].
 // With cb class bits its tiles are grouped by class p = H & (2^cb - 1), in

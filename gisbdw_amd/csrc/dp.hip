@@ -31,29 +31,81 @@
 namespace tdhip {
 namespace {
 
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kCbStride = 33;  // s_cb[x * 33 + j] = C(x, j), x < 32, j <= 32
-constexpr int kNbUnroll = 4;   // neighbour tiles loaded per step
+// The Pascal buffer continues with the offsets O_k = sum_{j<k} C(16, j)
+// (k = 0..16) and, from word kTailWord, the 16-bit colex unrank table of the
+// k-subsets of {0..15}: entry O_k + r is the subset of rank r.
+constexpr int kTailBits = 16;
+constexpr int kCbOffs = 32 * kCbStride;              // O_k at s_cb[kCbOffs + k]
+constexpr int kCbWords = kCbOffs + kTailBits + 1;    // LDS copy of the level kernels
+constexpr int kTailWord = (kCbWords + 31) / 32 * 32; // table start (128-byte aligned)
 
-__device__ __forceinline__ u16x2 AsU16x2(uint32_t x) {
-  return __builtin_bit_cast(u16x2, x);
+// Folding bytes with packed f16 min / max.  A table byte b (0..65) is put
+// in the HIGH byte of a 16-bit lane, over an arbitrary low byte (another
+// table byte, or 0): the pattern b:lo is a non-negative, finite f16 (b < 0x7C,
+// so the exponent is never all ones; b < 4 gives a denormal, and the kernels
+// keep f16 denormals: float_denorm_mode_16_64 = 3, the default).  Non-negative
+// f16 values order like their bit patterns, so the min / max of such lanes has
+// in its high byte the min / max of their high bytes, whatever the low bytes
+// were, and min / max return one of their inputs bit for bit.  A raw table
+// dword holds bytes 1 and 3 in those positions as loaded; bytes 0 and 2 get
+// there with one shift.  gfx950's three-input v_pk_minimum3_f16 /
+// v_pk_maximum3_f16 then fold two neighbour tiles per instruction.
+constexpr uint32_t kMinSentinel = 0x7B007B00u;  // high bytes 123 > any byte
+__device__ __forceinline__ f16x2 AsF16x2(uint32_t x) {
+  return __builtin_bit_cast(f16x2, x);
+}
+__device__ __forceinline__ f16x2 Min3(f16x2 a, f16x2 b, f16x2 c) {
+  return __builtin_elementwise_minimum(__builtin_elementwise_minimum(a, b), c);
+}
+__device__ __forceinline__ f16x2 Max3(f16x2 a, f16x2 b, f16x2 c) {
+  return __builtin_elementwise_maximum(__builtin_elementwise_maximum(a, b), c);
+}
+// bytes 0 and 2 (E) / 1 and 3 (O) of a dword in the high bytes of the lanes
+__device__ __forceinline__ f16x2 EvenBytes(uint32_t x) { return AsF16x2(x << 8); }
+__device__ __forceinline__ f16x2 OddBytes(uint32_t x) { return AsF16x2(x); }
+// high byte of lane `hi` of a folded pair
+__device__ __forceinline__ uint32_t HighByte(f16x2 v, bool hi) {
+  const uint32_t x = __builtin_bit_cast(uint32_t, v);
+  return hi ? x >> 24 : (x >> 8) & 0xFFu;
 }
 
 // Colex unrank of r among the k-subsets of {0..nh-1}; greedy from the top
-// (set_encoder.hpp:61-73) with the binomials read from LDS.
+// (set_encoder.hpp:61-73) with the binomials read from LDS.  Needs
+// r < C(nh, k): the remainder then reaches 0 exactly when k does, and
+// C(x, 0) = 1 > 0 takes nothing more, so no k > 0 test is needed.  H is
+// built MSB first (H = 2H + take), which keeps the loop to a compare, a
+// select and two carry-style updates per x.
 __device__ __forceinline__ uint32_t Unrank(uint32_t r, int k, int nh,
                                            const uint32_t* s_cb) {
   uint32_t H = 0;
   for (int x = nh - 1; x >= 0; --x) {
     const uint32_t c = s_cb[x * kCbStride + k];
-    if (k > 0 && r >= c) {
-      H |= 1u << x;
-      r -= c;
-      --k;
-    }
+    const uint32_t take = r >= c ? 1u : 0u;
+    r = take ? r - c : r;
+    H = 2u * H + take;
+    k -= static_cast<int>(take);
   }
   return H;
+}
+
+// Unrank for the level kernels: the greedy scan down to vertex 16, then
+// one read of the 16-bit table for the rest (the remainder is then below
+// C(16, k)).  Same result as Unrank; r < C(nh, k), nh <= 32.
+__device__ __forceinline__ uint32_t UnrankTiles(uint32_t r, int k, int nh,
+                                                const uint32_t* s_cb,
+                                                const uint16_t* __restrict__ tail) {
+  uint32_t H = 0;
+  for (int x = nh - 1; x >= kTailBits; --x) {
+    const uint32_t c = s_cb[x * kCbStride + k];
+    const uint32_t take = r >= c ? 1u : 0u;
+    r = take ? r - c : r;
+    H = 2u * H + take;
+    k -= static_cast<int>(take);
+  }
+  return (H << kTailBits) | tail[s_cb[kCbOffs + k] + r];
 }
 
 // Next k-subset in colex order (Gosper's hack, shift instead of divide).
@@ -114,151 +166,217 @@ __device__ __forceinline__ void DPTile(const uint8_t* __restrict__ prev,
                                        uint32_t Hr, int k, uint32_t H, int h,
                                        const uint8_t* const (&x)[4], int nx,
                                        const uint32_t* s_cb,
-                                       const uint32_t* s_nbh,
-                                       const uint32_t (&nl)[1 << LB]) {
+                                       const uint32_t* s_tab) {
   constexpr int NB = 1 << LB;
   constexpr int NW = NB / 4;
-  // byte-wise min / max of the neighbour tiles, kept as u16 pairs: E holds
-  // bytes 0 and 2 of each dword, O bytes 1 and 3; orr ORs raw bytes (bit 0
-  // of each byte = "some S \ x is connected").
-  u16x2 mnE[NW], mnO[NW], mxE[NW], mxO[NW];
+  // byte-wise min / max of the neighbour tiles, kept as 16-bit lane pairs
+  // (value in the high byte of each lane): E holds bytes 0 and 2 of each
+  // dword, O bytes 1 and 3; orr ORs raw bytes (bit 0 of each byte = "some
+  // S \ x is connected").
+  f16x2 mnE[NW], mnO[NW], mxE[NW], mxO[NW];
   uint32_t orr[NW];
   if constexpr (kTiny) {
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
-      mnE[w] = mnO[w] = AsU16x2(0x007F007Fu);
-      mxE[w] = mxO[w] = AsU16x2(0u);
+      mnE[w] = mnO[w] = AsF16x2(kMinSentinel);
+      mxE[w] = mxO[w] = AsF16x2(0u);
       orr[w] = 0u;
     }
   }
   // neighbour tiles Hr \ y_j, j = k-1 .. 0 (y_j = j-th smallest of Hr):
   // rank(Hr \ y_j) = t - C(y_j, j+1) - sum_{i>j} (C(y_i, i+1) - C(y_i, i)),
   // then (class layout) the nx tiles of the classes p \ b at rank t.
-  // Four neighbour tiles per step: k and nx are uniform across the wave, so
-  // the guards below are uniform and the four loads are in flight together.
+  // k and nx are uniform across the wave, so every guard below is uniform.
+  // Up to four loads are issued before their tiles are folded, in pairs
+  // (three-input min / max, three-input or).
+  // B = t - sum_{i>j} (C(y_i, i+1) - C(y_i, i)) runs down with j.  Byte
+  // offsets within a level fit 32 bits (a level is at most C(32,16) bytes),
+  // so the loads use a uniform base plus a 32-bit lane offset.
   uint32_t rem = Hr;
-  uint32_t A = 0;
+  uint32_t B = t;
   const int total = kClass ? k + nx : k;
-  for (int i0 = 0; i0 < total; i0 += kNbUnroll) {
-    uint32_t wv[kNbUnroll][NW];
-#pragma unroll
-    for (int u = 0; u < kNbUnroll; ++u) {
-      const int i = i0 + u;
-      if (i < k) {
-        const int j = k - 1 - i;
-        const int y = 31 - __builtin_clz(rem);
-        rem ^= 1u << y;
-        const uint32_t c1 = s_cb[y * kCbStride + j + 1];
-        const uint32_t c0 = s_cb[y * kCbStride + j];
-        const uint32_t r2 = t - c1 - A;
-        A += c1 - c0;
-        LoadTile<LB>(prev + (static_cast<size_t>(r2) << LB), wv[u]);
-      } else if (kClass && i < total) {
-        const int c = i - k;
-        const uint8_t* base = c == 0 ? x[0] : c == 1 ? x[1] : c == 2 ? x[2] : x[3];
-        LoadTile<LB>(base + (static_cast<size_t>(t) << LB), wv[u]);
-      }
+  auto load = [&](int i, uint32_t (&wv)[NW]) {
+    if (!kClass || i < k) {
+      const int j = k - 1 - i;
+      const int y = 31 - __builtin_clz(rem);
+      rem ^= 1u << y;
+      const uint32_t c1 = s_cb[y * kCbStride + j + 1];
+      const uint32_t c0 = s_cb[y * kCbStride + j];
+      const uint32_t r2 = B - c1;
+      B = r2 + c0;
+      LoadTile<LB>(prev + (r2 << LB), wv);
+    } else {
+      const int c = i - k;
+      const uint8_t* base = c == 0 ? x[0] : c == 1 ? x[1] : c == 2 ? x[2] : x[3];
+      LoadTile<LB>(base + (t << LB), wv);
     }
+  };
+  auto fold2 = [&](const uint32_t (&wa)[NW], const uint32_t (&wb)[NW]) {
 #pragma unroll
-    for (int u = 0; u < kNbUnroll; ++u) {
-      if (i0 + u < total) {
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t a = wa[w], b = wb[w];
+      mnE[w] = Min3(mnE[w], EvenBytes(a), EvenBytes(b));
+      mnO[w] = Min3(mnO[w], OddBytes(a), OddBytes(b));
+      mxE[w] = Max3(mxE[w], EvenBytes(a), EvenBytes(b));
+      mxO[w] = Max3(mxO[w], OddBytes(a), OddBytes(b));
+      orr[w] = orr[w] | a | b;
+    }
+  };
+  int i = 0;
+  if constexpr (!kTiny) {
+    // h >= 2, so at least two neighbour tiles: the first two start the fold
+    uint32_t wa[NW], wb[NW];
+    load(0, wa);
+    load(1, wb);
 #pragma unroll
-        for (int w = 0; w < NW; ++w) {
-          const uint32_t xw = wv[u][w];
-          const u16x2 xe = AsU16x2(__builtin_amdgcn_perm(xw, xw, 0x0c020c00u));
-          const u16x2 xo = AsU16x2(__builtin_amdgcn_perm(xw, xw, 0x0c030c01u));
-          if (!kTiny && u == 0 && i0 == 0) {
-            // h >= 2: the first neighbour tile starts the fold
-            mnE[w] = mxE[w] = xe;
-            mnO[w] = mxO[w] = xo;
-            orr[w] = xw;
-          } else {
-            mnE[w] = __builtin_elementwise_min(mnE[w], xe);
-            mnO[w] = __builtin_elementwise_min(mnO[w], xo);
-            mxE[w] = __builtin_elementwise_max(mxE[w], xe);
-            mxO[w] = __builtin_elementwise_max(mxO[w], xo);
-            orr[w] |= xw;
-          }
-        }
-      }
+    for (int w = 0; w < NW; ++w) {
+      const f16x2 ae = EvenBytes(wa[w]), ao = OddBytes(wa[w]);
+      const f16x2 be = EvenBytes(wb[w]), bo = OddBytes(wb[w]);
+      mnE[w] = __builtin_elementwise_minimum(ae, be);
+      mnO[w] = __builtin_elementwise_minimum(ao, bo);
+      mxE[w] = __builtin_elementwise_maximum(ae, be);
+      mxO[w] = __builtin_elementwise_maximum(ao, bo);
+      orr[w] = wa[w] | wb[w];
+    }
+    i = 2;
+  }
+  for (; i + 4 <= total; i += 4) {
+    uint32_t w0[NW], w1[NW], w2[NW], w3[NW];
+    load(i, w0);
+    load(i + 1, w1);
+    load(i + 2, w2);
+    load(i + 3, w3);
+    fold2(w0, w1);
+    fold2(w2, w3);
+  }
+  if (i + 2 <= total) {
+    uint32_t w0[NW], w1[NW];
+    load(i, w0);
+    load(i + 1, w1);
+    fold2(w0, w1);
+    i += 2;
+  }
+  if (i < total) {
+    uint32_t w0[NW];
+    load(i, w0);
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      mnE[w] = __builtin_elementwise_minimum(mnE[w], EvenBytes(w0[w]));
+      mnO[w] = __builtin_elementwise_minimum(mnO[w], OddBytes(w0[w]));
+      mxE[w] = __builtin_elementwise_maximum(mxE[w], EvenBytes(w0[w]));
+      mxO[w] = __builtin_elementwise_maximum(mxO[w], OddBytes(w0[w]));
+      orr[w] |= w0[w];
     }
   }
-  // N(H) from the 7-bit chunk tables
-  const uint32_t NH = s_nbh[H & 127u] | s_nbh[128 + ((H >> 7) & 127u)] |
-                      s_nbh[256 + ((H >> 14) & 127u)] |
-                      s_nbh[384 + ((H >> 21) & 127u)];
-  const uint32_t Sh = H << LB;
+  // Connectivity.  S = H u L is connected iff no vertex of S is isolated in S
+  // and some S \ v is connected (or S is a single vertex).  The first part
+  // is one mask per tile, F, over the 2^LB low parts, in the tile-mask bit
+  // order (bit 8 (L & 3) + (L >> 2) for low part L, so that F >> w holds the
+  // four low parts of dword w in bit 0 of its bytes):
+  //   high x in H with no neighbour in H needs a neighbour in L;
+  //   low y in L with no neighbour in H needs a neighbour in L;
+  // both from per-graph tables (7-bit chunks of the isolated high vertices,
+  // and the set of low vertices H does not reach).
+  const uint32_t NH = s_tab[kTabNbh + (H & 127u)] |
+                      s_tab[kTabNbh + 128 + ((H >> 7) & 127u)] |
+                      s_tab[kTabNbh + 256 + ((H >> 14) & 127u)] |
+                      s_tab[kTabNbh + 384 + ((H >> 21) & 127u)];
+  const uint32_t IH = H & ~(NH >> LB);
+  const uint32_t W = ~NH & ((1u << LB) - 1u);
+  const uint32_t F = s_tab[kTabIsoL + W] & s_tab[kTabIsoH + (IH & 127u)] &
+                     s_tab[kTabIsoH + 128 + ((IH >> 7) & 127u)] &
+                     s_tab[kTabIsoH + 256 + ((IH >> 14) & 127u)] &
+                     s_tab[kTabIsoH + 384 + ((IH >> 21) & 127u)];
+  // The second part, bytewise: C[w] has bit 0 of byte b set iff low part
+  // 4w + b gives a connected S.  Predecessors of 4w + b: the same byte of
+  // the neighbour tiles (orr), the same byte of dwords w \ j (done before w),
+  // and inside the dword byte 0 -> bytes 1, 2 -> byte 3.
+  uint32_t C[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    uint32_t a = orr[w];
+#pragma unroll
+    for (int j = 0; j < LB - 2; ++j)
+      if (w >> j & 1) a |= C[w ^ (1 << j)];
+    uint32_t Fw = (F >> w) & 0x01010101u;
+    if constexpr (kTiny) {
+      // single vertices: L = {l} at h = 0 (bytes 1, 2 of dword 0 and byte 0
+      // of dwords 1, 2, 4), L = 0 at h = 1
+      uint32_t single = 0u;
+      if (h == 0) single = w == 0 ? 0x00010100u : (w & (w - 1)) == 0 ? 1u : 0u;
+      else if (w == 0) single = 1u;
+      Fw |= single;
+      a |= single;
+    }
+    a &= Fw;
+    a = (a | __builtin_amdgcn_perm(a, a, 0x0c00000cu)) & Fw;
+    a = (a | ((a << 8 | a << 16) & 0xFF000000u)) & Fw;
+    C[w] = a;
+  }
+  // Values, kept as 2 td + junk (bit 0 is left as min / max found it; only
+  // the td part orders): connected -> 1 + min, else max; bit 0 := C at the end.
   uint32_t V[NB];
 #pragma unroll
   for (int L = 0; L < NB; ++L) {
     const int w = L >> 2;
-    const u16x2 em = (L & 1) ? mnO[w] : mnE[w];
-    const u16x2 eM = (L & 1) ? mxO[w] : mxE[w];
-    uint32_t m = (L & 2) ? em.y : em.x;
-    uint32_t M = (L & 2) ? eM.y : eM.x;
-    uint32_t o = orr[w] >> (8 * (L & 3));
+    uint32_t m = HighByte((L & 1) ? mnO[w] : mnE[w], L & 2);
+    uint32_t M = HighByte((L & 1) ? mxO[w] : mxE[w], L & 2);
 #pragma unroll
     for (int l = 0; l < LB; ++l) {
       if (L >> l & 1) {
         const uint32_t v = V[L ^ (1 << l)];
         m = min(m, v);
         M = max(M, v);
-        o |= v;
       }
     }
-    const uint32_t S = Sh | static_cast<uint32_t>(L);
-    const uint32_t cover = NH | nl[L];
-    const bool conn = ((S & ~cover) == 0u) && (o & 1u);
-    uint32_t val = conn ? ((m & ~1u) + 3u) : (M & ~1u);
-    if (kTiny && __builtin_popcount(L) + h == 1) val = 3u;  // a single vertex
-    V[L] = val;
+    const bool conn = ((C[w] >> (8 * (L & 3))) & 0xFFu) != 0u;
+    V[L] = conn ? m + 2u : M;
   }
   uint32_t ow[NW];
 #pragma unroll
   for (int w = 0; w < NW; ++w)
-    ow[w] = V[4 * w] | (V[4 * w + 1] << 8) | (V[4 * w + 2] << 16) |
-            (V[4 * w + 3] << 24);
-  StoreTile<LB>(cur + (static_cast<size_t>(t) << LB), ow);
+    ow[w] = ((V[4 * w] | (V[4 * w + 1] << 8) | (V[4 * w + 2] << 16) |
+              (V[4 * w + 3] << 24)) & 0xFEFEFEFEu) | C[w];
+  StoreTile<LB>(cur + (t << LB), ow);
 }
 
 template <int LB>
 __device__ __forceinline__ void LoadShared(const DPGraph* __restrict__ g,
                                            const uint32_t* __restrict__ g_cb,
-                                           uint32_t* s_cb, uint32_t* s_nbh,
+                                           uint32_t* s_cb, uint32_t* s_tab,
                                            bool load_cb) {
   if (load_cb)
-    for (int i = threadIdx.x; i < 32 * kCbStride; i += blockDim.x)
-      s_cb[i] = g_cb[i];
-  for (int i = threadIdx.x; i < 512; i += blockDim.x) s_nbh[i] = g->nbh[i];
+    for (int i = threadIdx.x; i < kCbWords; i += blockDim.x) s_cb[i] = g_cb[i];
+  // nbh, isoh and isol are consecutive in DPGraph
+  for (int i = threadIdx.x; i < kTabWords; i += blockDim.x) s_tab[i] = g->nbh[i];
 }
 
 // Tiles [first, end) of high level h of graph g, R consecutive per lane,
 // lanes of the launch striding by `stride` tiles.
 template <int LB>
 __device__ __forceinline__ void DPLevelRange(const DPGraph* __restrict__ g,
+                                             const uint32_t* __restrict__ g_cb,
                                              uint8_t* __restrict__ arena,
                                              int h, uint32_t first,
                                              uint32_t end, uint32_t lane_id,
                                              uint32_t stride, int R,
                                              const uint32_t* s_cb,
-                                             const uint32_t* s_nbh) {
-  uint32_t nl[1 << LB];
-#pragma unroll
-  for (int L = 0; L < (1 << LB); ++L) nl[L] = g->nl[L];
+                                             const uint32_t* s_tab) {
   const int nh = static_cast<int>(g->nh);
   uint8_t* base = arena + g->table;
   const uint8_t* prev =
       h > 0 ? base + (static_cast<size_t>(g->lvl[h - 1]) << LB) : base;
   uint8_t* cur = base + (static_cast<size_t>(g->lvl[h]) << LB);
   const uint8_t* const none[4] = {nullptr, nullptr, nullptr, nullptr};
+  const uint16_t* tail = reinterpret_cast<const uint16_t*>(g_cb + kTailWord);
   for (uint32_t t0 = first + lane_id * R; t0 < end; t0 += stride) {
-    uint32_t H = Unrank(t0, h, nh, s_cb);
+    uint32_t H = UnrankTiles(t0, h, nh, s_cb, tail);
     const uint32_t tend = min(t0 + static_cast<uint32_t>(R), end);
     for (uint32_t t = t0;;) {
       if (h <= 1)
-        DPTile<LB, false, true>(prev, cur, t, H, h, H, h, none, 0, s_cb, s_nbh, nl);
+        DPTile<LB, false, true>(prev, cur, t, H, h, H, h, none, 0, s_cb, s_tab);
       else
-        DPTile<LB, false, false>(prev, cur, t, H, h, H, h, none, 0, s_cb, s_nbh, nl);
+        DPTile<LB, false, false>(prev, cur, t, H, h, H, h, none, 0, s_cb, s_tab);
       if (++t >= tend) break;
       H = NextCombination(H);
     }
@@ -276,16 +394,16 @@ template <int LB>
 __global__ __launch_bounds__(1024) void DPLevelKernel(
     const DPGraph* __restrict__ g, const uint32_t* __restrict__ g_cb,
     uint8_t* __restrict__ arena, int h, uint32_t ntiles, int R, int xcd_remap) {
-  __shared__ uint32_t s_cb[32 * kCbStride];
-  __shared__ uint32_t s_nbh[512];
-  LoadShared<LB>(g, g_cb, s_cb, s_nbh, true);
+  __shared__ uint32_t s_cb[kCbWords];
+  __shared__ uint32_t s_tab[kTabWords];
+  LoadShared<LB>(g, g_cb, s_cb, s_tab, true);
   __syncthreads();
   uint32_t bid = blockIdx.x;
   if (xcd_remap && gridDim.x % kXcds == 0)
     bid = (bid % kXcds) * (gridDim.x / kXcds) + bid / kXcds;
   const uint32_t lane_id = bid * blockDim.x + threadIdx.x;
   const uint32_t stride = gridDim.x * blockDim.x * R;
-  DPLevelRange<LB>(g, arena, h, 0, ntiles, lane_id, stride, R, s_cb, s_nbh);
+  DPLevelRange<LB>(g, g_cb, arena, h, 0, ntiles, lane_id, stride, R, s_cb, s_tab);
 }
 
 // Static schedule of one class-layout level launch: for each XCD, up to
@@ -313,18 +431,16 @@ template <int LB>
 __global__ __launch_bounds__(1024) void DPClassLevelKernel(
     const DPGraph* __restrict__ g, const uint32_t* __restrict__ g_cb,
     uint8_t* __restrict__ arena, int h, int R, const ClassSched sched) {
-  __shared__ uint32_t s_cb[32 * kCbStride];
-  __shared__ uint32_t s_nbh[512];
-  LoadShared<LB>(g, g_cb, s_cb, s_nbh, true);
+  __shared__ uint32_t s_cb[kCbWords];
+  __shared__ uint32_t s_tab[kTabWords];
+  LoadShared<LB>(g, g_cb, s_cb, s_tab, true);
   __syncthreads();
   const int cb = static_cast<int>(g->cb);
   const int rest = static_cast<int>(g->nh) - cb;
+  const uint16_t* tail = reinterpret_cast<const uint16_t*>(g_cb + kTailWord);
   const uint32_t CH = blockDim.x * static_cast<uint32_t>(R);
   const uint32_t xcd = blockIdx.x % kXcds;
   const uint32_t wx = blockIdx.x / kXcds, gx = gridDim.x / kXcds;
-  uint32_t nl[1 << LB];
-#pragma unroll
-  for (int L = 0; L < (1 << LB); ++L) nl[L] = g->nl[L];
   uint8_t* base = arena + g->table;
   const uint32_t nseg = sched.nseg[xcd];
   for (uint32_t s = 0; s < nseg; ++s) {
@@ -341,14 +457,14 @@ __global__ __launch_bounds__(1024) void DPClassLevelKernel(
     for (uint32_t c0 = sg.first + wx * CH; c0 < sg.end; c0 += gx * CH) {
       const uint32_t t0 = c0 + threadIdx.x * static_cast<uint32_t>(R);
       if (t0 >= sg.end) continue;
-      uint32_t Hr = Unrank(t0, k, rest, s_cb);
+      uint32_t Hr = UnrankTiles(t0, k, rest, s_cb, tail);
       const uint32_t tend = min(t0 + static_cast<uint32_t>(R), sg.end);
       for (uint32_t t = t0;;) {
         const uint32_t H = p | (Hr << cb);
         if (h <= 1)
-          DPTile<LB, true, true>(prev, cur, t, Hr, k, H, h, x, np, s_cb, s_nbh, nl);
+          DPTile<LB, true, true>(prev, cur, t, Hr, k, H, h, x, np, s_cb, s_tab);
         else
-          DPTile<LB, true, false>(prev, cur, t, Hr, k, H, h, x, np, s_cb, s_nbh, nl);
+          DPTile<LB, true, false>(prev, cur, t, Hr, k, H, h, x, np, s_cb, s_tab);
         if (++t >= tend) break;
         Hr = NextCombination(Hr);
       }
@@ -367,14 +483,14 @@ __global__ __launch_bounds__(1024) void DPChunkKernel(
     const DPGraph* __restrict__ gs, const uint32_t* __restrict__ g_cb,
     uint8_t* __restrict__ arena, const Chunk* __restrict__ chunks, int h,
     int R) {
-  __shared__ uint32_t s_cb[32 * kCbStride];
-  __shared__ uint32_t s_nbh[512];
+  __shared__ uint32_t s_cb[kCbWords];
+  __shared__ uint32_t s_tab[kTabWords];
   const Chunk c = chunks[blockIdx.x];
   const DPGraph* g = gs + c.gi;
-  LoadShared<LB>(g, g_cb, s_cb, s_nbh, true);
+  LoadShared<LB>(g, g_cb, s_cb, s_tab, true);
   __syncthreads();
-  DPLevelRange<LB>(g, arena, h, c.first, c.end, threadIdx.x,
-                   blockDim.x * R, R, s_cb, s_nbh);
+  DPLevelRange<LB>(g, g_cb, arena, h, c.first, c.end, threadIdx.x,
+                   blockDim.x * R, R, s_cb, s_tab);
 }
 
 // Persistent batch: workgroups pull whole graphs from a device queue (graphs
@@ -386,24 +502,23 @@ __global__ __launch_bounds__(1024) void DPPersistentKernel(
     const DPGraph* __restrict__ gs, const uint32_t* __restrict__ g_cb,
     uint8_t* __restrict__ arena, int ngraphs, uint32_t* __restrict__ queue,
     int R) {
-  __shared__ uint32_t s_cb[32 * kCbStride];
-  __shared__ uint32_t s_nbh[512];
+  __shared__ uint32_t s_cb[kCbWords];
+  __shared__ uint32_t s_tab[kTabWords];
   __shared__ int s_gi;
-  for (int i = threadIdx.x; i < 32 * kCbStride; i += blockDim.x)
-    s_cb[i] = g_cb[i];
+  for (int i = threadIdx.x; i < kCbWords; i += blockDim.x) s_cb[i] = g_cb[i];
   for (;;) {
     if (threadIdx.x == 0) s_gi = static_cast<int>(atomicAdd(queue, 1u));
     __syncthreads();
     const int gi = s_gi;
     if (gi >= ngraphs) break;  // uniform: every thread read the same s_gi
     const DPGraph* g = gs + gi;
-    LoadShared<LB>(g, g_cb, s_cb, s_nbh, false);
+    LoadShared<LB>(g, g_cb, s_cb, s_tab, false);
     __syncthreads();
     const int nh = static_cast<int>(g->nh);
     for (int h = 0; h <= nh; ++h) {
       const uint32_t ntiles = s_cb[nh * kCbStride + h];  // C(nh, h), no padding
-      DPLevelRange<LB>(g, arena, h, 0, ntiles, threadIdx.x, blockDim.x * R, R,
-                       s_cb, s_nbh);
+      DPLevelRange<LB>(g, g_cb, arena, h, 0, ntiles, threadIdx.x, blockDim.x * R, R,
+                       s_cb, s_tab);
       __syncthreads();
     }
   }
@@ -416,14 +531,14 @@ template <int LB>
 __global__ __launch_bounds__(1024) void DPSmallLevelsKernel(
     const DPGraph* __restrict__ g, const uint32_t* __restrict__ g_cb,
     uint8_t* __restrict__ arena, int h0, int h1) {
-  __shared__ uint32_t s_cb[32 * kCbStride];
-  __shared__ uint32_t s_nbh[512];
-  LoadShared<LB>(g, g_cb, s_cb, s_nbh, true);
+  __shared__ uint32_t s_cb[kCbWords];
+  __shared__ uint32_t s_tab[kTabWords];
+  LoadShared<LB>(g, g_cb, s_cb, s_tab, true);
   __syncthreads();
   const int nh = static_cast<int>(g->nh);
   for (int h = h0; h <= h1; ++h) {
     const uint32_t ntiles = s_cb[nh * kCbStride + h];
-    DPLevelRange<LB>(g, arena, h, 0, ntiles, threadIdx.x, blockDim.x, 1, s_cb, s_nbh);
+    DPLevelRange<LB>(g, g_cb, arena, h, 0, ntiles, threadIdx.x, blockDim.x, 1, s_cb, s_tab);
     __syncthreads();
   }
 }
@@ -620,6 +735,34 @@ DPGraph MakeGraph(int n, const uint64_t* adj, int lb, uint64_t table_off, int cb
       G.nbh[128 * c + chunk] = G.nbh[128 * c + (chunk & (chunk - 1))] | row;
     }
   }
+  // isolation masks over the low parts L (tile-mask bit order)
+  const int NB = 1 << lb;
+  auto bit = [](int L) { return 1u << (8 * (L & 3) + (L >> 2)); };
+  const uint32_t lowmask = (1u << lb) - 1u;
+  for (int c = 0; c < 4; ++c) {
+    uint32_t all = 0;
+    for (int L = 0; L < NB; ++L) all |= bit(L);
+    G.isoh[128 * c] = all;
+    for (int chunk = 1; chunk < 128; ++chunk) {
+      const int b = __builtin_ctz(static_cast<unsigned>(chunk));
+      const int x = 7 * c + b;
+      const uint32_t low = x < static_cast<int>(G.nh) ? G.adj[lb + x] & lowmask : 0u;
+      uint32_t reach = 0;  // the L meeting N(x)
+      for (int L = 0; L < NB; ++L)
+        if (static_cast<uint32_t>(L) & low) reach |= bit(L);
+      G.isoh[128 * c + chunk] = G.isoh[128 * c + (chunk & (chunk - 1))] & reach;
+    }
+  }
+  for (int Y = 0; Y < NB; ++Y) {
+    uint32_t m = 0;
+    for (int L = 0; L < NB; ++L) {
+      bool ok = true;
+      for (int y = 0; y < lb; ++y)
+        if ((Y & L) >> y & 1 && (G.adj[y] & static_cast<uint32_t>(L)) == 0u) ok = false;
+      if (ok) m |= bit(L);
+    }
+    G.isol[Y] = m;
+  }
   return G;
 }
 
@@ -631,10 +774,19 @@ const uint32_t* PascalDevice(td_ctx* ctx) {
   auto it = ctx->bufs.find("dp.pascal");
   if (it != ctx->bufs.end() && it->second.p)
     return static_cast<const uint32_t*>(it->second.p);
-  std::vector<uint32_t> cb(32 * kCbStride, 0);
+  // binomials, O_k, then the 16-bit unrank table (2 entries per word)
+  std::vector<uint32_t> cb(kTailWord + (1u << kTailBits) / 2, 0);
   for (int x = 0; x < 32; ++x)
     for (int j = 0; j < kCbStride; ++j)
       cb[x * kCbStride + j] = static_cast<uint32_t>(Binom(x, j));
+  uint16_t* tail = reinterpret_cast<uint16_t*>(cb.data() + kTailWord);
+  uint32_t off = 0;
+  for (int k = 0; k <= kTailBits; ++k) {
+    cb[kCbOffs + k] = off;
+    // the k-subsets of {0..15} in colex order = increasing integers
+    for (uint32_t m = 0; m < (1u << kTailBits); ++m)
+      if (__builtin_popcount(m) == k) tail[off++] = static_cast<uint16_t>(m);
+  }
   void* d = ctx->Buffer("dp.pascal", cb.size() * 4);
   TD_HIP(hipMemcpy(d, cb.data(), cb.size() * 4, hipMemcpyHostToDevice));
   return static_cast<const uint32_t*>(d);
