@@ -10,13 +10,17 @@ over the timed steps (wall clock; every call is synchronous).
 roofline: the level kernels against HBM peak, with achieved = algorithmic
 bytes / GPU time of the level launches (HIP events on the library's stream
 around each graph's level sequence), algorithmic bytes = 2^(n-1) (n+2) per
-graph (k one-byte reads of level k-1 and one write per k-subset).  A graph's
-levels run as DPLevelKernel launches, except the runs of small levels at
-either end, which one DPSmallLevelsKernel workgroup does per run; per-launch
-figures divide by that launch count.
+graph (k one-byte reads of level k-1 and one write per k-subset).  In the
+plain layout a graph's levels run as DPLevelKernel launches, except the runs
+of small levels at either end, which one DPSmallLevelsKernel workgroup does
+per run; in the class layout every level is one DPClassLevelKernel launch.
+Per-launch figures divide by that launch count.  traffic comes from the
+committed rocprof summary line for the launch shape that ran (null if the
+summary has none).
 
-Launch shape: td_dp_autotune picks (low_bits, tiles_per_lane, block) for n
-before the warm-up, unless --no-autotune or an explicit shape flag is given.
+Launch shape: td_dp_autotune picks (class_bits, low_bits, tiles_per_lane,
+grid_mult, block) for n before the warm-up, unless --no-autotune or an
+explicit shape flag is given.
 
 cpu_baseline: the oracle's flat-table DP (OpenMP over each level) timed on the
 host on a bounded sample: the subgraph induced by the first m vertices of the
@@ -60,6 +64,7 @@ def parse():
     ap.add_argument("--block", type=int, default=None)
     ap.add_argument("--xcd-remap", type=int, default=None, choices=[0, 1])
     ap.add_argument("--class-bits", type=int, default=None, choices=[0, 3, 4])
+    ap.add_argument("--grid-mult", type=int, default=None)
     ap.add_argument("--batch-block", type=int, default=None)
     ap.add_argument("--batch-wg-per-cu", type=int, default=None)
     ap.add_argument("--no-autotune", action="store_true",
@@ -307,14 +312,23 @@ def cpu_extra(threads):
     return out
 
 
-def profile_traffic():
-    """HBM bytes per level launch from the committed rocprofv3 --pmc summary
-    (profiles/dp_level_kernel_n30_traffic.md, calibrated against a plain copy
-    of known size), or None when the summary is absent."""
+def shape_key(ctx):
+    """The DP launch shape, as the traffic summary keys its lines."""
+    return ("low_bits={} tiles_per_lane={} block={} class_bits={} grid_mult={}".format(
+        *(ctx.get_param("dp." + k) for k in
+          ("low_bits", "tiles_per_lane", "block", "class_bits", "grid_mult"))))
+
+
+def profile_traffic(key):
+    """HBM bytes per level launch for this launch shape from the committed
+    rocprofv3 --pmc summary (profiles/dp_level_kernel_n30_traffic.md,
+    calibrated against a plain copy of known size), or None when the summary
+    has no line for the shape."""
     p = ROOT / "profiles" / "dp_level_kernel_n30_traffic.md"
     if p.exists():
+        want = f"traffic_bytes_per_launch [{key}]:"
         for line in p.read_text().splitlines():
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(want):
                 return int(line.split(":")[1])
     return None
 
@@ -344,21 +358,21 @@ def main():
         ctx.set_param("dp.xcd_remap", args.xcd_remap)
     if args.class_bits is not None:
         ctx.set_param("dp.class_bits", args.class_bits)
+    if args.grid_mult:
+        ctx.set_param("dp.grid_mult", args.grid_mult)
     if args.batch_block:
         ctx.set_param("dp.batch_block", args.batch_block)
     if args.batch_wg_per_cu:
         ctx.set_param("dp.batch_wg_per_cu", args.batch_wg_per_cu)
 
     explicit = args.low_bits or args.tiles_per_lane or args.block or args.batch_block or \
-        args.batch_wg_per_cu or args.class_bits is not None
+        args.batch_wg_per_cu or args.class_bits is not None or args.grid_mult
     tuned_ms = None
     if args.workload in ("dp30", "dp20"):
         graphs = G.workload_dp_n30() if args.workload == "dp30" else [G.workload_dp_single(0)]
         if not explicit and not args.no_autotune:
             tuned_ms = ctx.autotune(graphs[0].n)
-            print(f"[bench] autotuned: low_bits={ctx.get_param('dp.low_bits')} "
-                  f"tiles_per_lane={ctx.get_param('dp.tiles_per_lane')} "
-                  f"block={ctx.get_param('dp.block')}", file=sys.stderr, flush=True)
+            print(f"[bench] autotuned: {shape_key(ctx)}", file=sys.stderr, flush=True)
         dp = api.DynamicGPU(ctx)
 
         kernels = [0]
@@ -417,6 +431,7 @@ def main():
                 "block": ctx.get_param("dp.block"),
                 "xcd_remap": ctx.get_param("dp.xcd_remap"),
                 "class_bits": ctx.get_param("dp.class_bits"),
+                "grid_mult": ctx.get_param("dp.grid_mult"),
                 "autotuned": tuned_ms is not None,
             },
             "roofline": {
@@ -425,8 +440,9 @@ def main():
                 "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBPS,
-                "traffic": profile_traffic(),
-                "kernel": "DPLevelKernel",
+                "traffic": profile_traffic(shape_key(ctx)),
+                "kernel": ("DPClassLevelKernel" if ctx.get_param("dp.class_bits")
+                           else "DPLevelKernel"),
                 "alg_bytes_per_launch": alg / nlev,
                 "avg_launch_ms": lvl_ms / nlev,
                 "launches": nlev,

@@ -238,6 +238,9 @@ int td_ctx_set_param(td_ctx* c, const char* key, int64_t v) {
   } else if (k == "dp.class_min_nh") {
     if (v < 5 || v > 32) return bad("must be in [5, 32]");
     c->dp.class_min_nh = (int)v;
+  } else if (k == "dp.grid_mult") {
+    if (v < 1 || v > 64) return bad("must be in [1, 64]");
+    c->dp.grid_mult = (int)v;
   } else if (k.rfind("bnb.", 0) == 0 || k.rfind("mlp.", 0) == 0) {
     c->params[k] = v;
   } else {
@@ -261,6 +264,7 @@ int td_ctx_get_param(const td_ctx* c, const char* key, int64_t* v) {
   else if (k == "dp.xcd_remap") *v = c->dp.xcd_remap;
   else if (k == "dp.class_bits") *v = c->dp.class_bits;
   else if (k == "dp.class_min_nh") *v = c->dp.class_min_nh;
+  else if (k == "dp.grid_mult") *v = c->dp.grid_mult;
   else {
     auto it = c->params.find(k);
     if (it == c->params.end()) return TD_E_INVALID;

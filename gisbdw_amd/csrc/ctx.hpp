@@ -92,15 +92,16 @@ __host__ __device__ inline uint32_t TileIndex(uint32_t H, int nh, int cb, BinomF
 
 // DP launch configuration (autotuned).
 struct DPConfig {
-  int low_bits = 5;        // 2..5 (autotuned on MI355X at n = 30)
+  int low_bits = 4;        // 2..5 (autotuned on MI355X at n = 30)
   int tiles_per_lane = 1;  // consecutive tiles per lane (Gosper steps)
-  int block = 256;         // threads per workgroup
+  int block = 512;         // threads per workgroup
   int batch_wg_per_cu = 4; // persistent batch kernel: workgroups per CU
   int batch_block = 1024;  // persistent batch kernel: threads per workgroup
   int use_graph = 1;       // capture the level sequence into a hipGraph
   int xcd_remap = 0;       // contiguous tile slice per XCD (measured slower)
   int class_bits = 0;      // single-graph level layout: 0 (plain colex), 3 or 4
   int class_min_nh = 16;   // class layout only from this many high vertices
+  int grid_mult = 4;       // level launch: at most grid_mult x the resident workgroups
 };
 
 struct DeviceBuffer {

@@ -34,13 +34,22 @@ namespace {
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kCbStride = 33;  // s_cb[x * 33 + j] = C(x, j), x < 32, j <= 32
-// The Pascal buffer continues with the offsets O_k = sum_{j<k} C(16, j)
-// (k = 0..16) and, from word kTailWord, the 16-bit colex unrank table of the
-// k-subsets of {0..15}: entry O_k + r is the subset of rank r.
+// The Pascal buffer (global) holds: C(x, j) at word x * 33 + j; the offsets
+// O_k = sum_{j<k} C(16, j) (k = 0..16) at word kCbOffs + k; from word
+// kTailWord the 16-bit colex unrank table of the k-subsets of {0..15} (entry
+// O_k + r is the subset of rank r); from word kRevWord, C(31 - z, j) at
+// z * 33 + j.
 constexpr int kTailBits = 16;
-constexpr int kCbOffs = 32 * kCbStride;              // O_k at s_cb[kCbOffs + k]
-constexpr int kCbWords = kCbOffs + kTailBits + 1;    // LDS copy of the level kernels
-constexpr int kTailWord = (kCbWords + 31) / 32 * 32; // table start (128-byte aligned)
+constexpr int kCbOffs = 32 * kCbStride;
+constexpr int kTailWord = (kCbOffs + kTailBits + 1 + 31) / 32 * 32;
+constexpr int kRevWord = kTailWord + (1 << kTailBits) / 2;
+constexpr int kPascalWords = kRevWord + 32 * kCbStride;
+// The level kernels keep in LDS (s_lvl) C(31 - z, j) << LB at z * 33 + j,
+// indexed by z = leading zeros of a vertex bit, so that neighbour ranks come
+// out as byte offsets, then O_k at kLvlOffs + k.  Every entry read is at
+// most a level's byte count, below 2^32; entries never read may wrap.
+constexpr int kLvlOffs = 32 * kCbStride;
+constexpr int kLvlWords = kLvlOffs + kTailBits + 1;
 
 // Folding bytes with packed f16 min / max.  A table byte b (0..65) is put
 // in the HIGH byte of a 16-bit lane, over an arbitrary low byte (another
@@ -91,21 +100,30 @@ __device__ __forceinline__ uint32_t Unrank(uint32_t r, int k, int nh,
   return H;
 }
 
-// Unrank for the level kernels: the greedy scan down to vertex 16, then
-// one read of the 16-bit table for the rest (the remainder is then below
-// C(16, k)).  Same result as Unrank; r < C(nh, k), nh <= 32.
+// Unrank for the level kernels, on the shifted table: the greedy scan down
+// to vertex 16 with r and the binomials scaled by 2^LB, then one read of the
+// 16-bit table for the rest (the remainder is then below C(16, k)).  Same
+// result as Unrank; r < C(nh, k), nh <= 32.
+template <int LB>
 __device__ __forceinline__ uint32_t UnrankTiles(uint32_t r, int k, int nh,
-                                                const uint32_t* s_cb,
+                                                const uint32_t* s_lvl,
                                                 const uint16_t* __restrict__ tail) {
+  uint32_t rs = r << LB;
   uint32_t H = 0;
-  for (int x = nh - 1; x >= kTailBits; --x) {
-    const uint32_t c = s_cb[x * kCbStride + k];
-    const uint32_t take = r >= c ? 1u : 0u;
-    r = take ? r - c : r;
+  for (int z = 32 - nh; z < 32 - kTailBits; ++z) {  // vertex 31 - z
+    const uint32_t c = s_lvl[z * kCbStride + k];
+    const uint32_t take = rs >= c ? 1u : 0u;
+    rs = take ? rs - c : rs;
     H = 2u * H + take;
     k -= static_cast<int>(take);
   }
-  return (H << kTailBits) | tail[s_cb[kCbOffs + k] + r];
+  return (H << kTailBits) | tail[s_lvl[kLvlOffs + k] + (rs >> LB)];
+}
+
+// C(nh, h) from the shifted table
+template <int LB>
+__device__ __forceinline__ uint32_t LevelTiles(const uint32_t* s_lvl, int nh, int h) {
+  return s_lvl[(31 - nh) * kCbStride + h] >> LB;
 }
 
 // Next k-subset in colex order (Gosper's hack, shift instead of divide).
@@ -165,7 +183,7 @@ __device__ __forceinline__ void DPTile(const uint8_t* __restrict__ prev,
                                        uint8_t* __restrict__ cur, uint32_t t,
                                        uint32_t Hr, int k, uint32_t H, int h,
                                        const uint8_t* const (&x)[4], int nx,
-                                       const uint32_t* s_cb,
+                                       const uint32_t* s_lvl,
                                        const uint32_t* s_tab) {
   constexpr int NB = 1 << LB;
   constexpr int NW = NB / 4;
@@ -189,22 +207,22 @@ __device__ __forceinline__ void DPTile(const uint8_t* __restrict__ prev,
   // k and nx are uniform across the wave, so every guard below is uniform.
   // Up to four loads are issued before their tiles are folded, in pairs
   // (three-input min / max, three-input or).
-  // B = t - sum_{i>j} (C(y_i, i+1) - C(y_i, i)) runs down with j.  Byte
+  // B = (t - sum_{i>j} (C(y_i, i+1) - C(y_i, i))) << LB runs down with j.  Byte
   // offsets within a level fit 32 bits (a level is at most C(32,16) bytes),
   // so the loads use a uniform base plus a 32-bit lane offset.
   uint32_t rem = Hr;
-  uint32_t B = t;
+  uint32_t B = t << LB;
   const int total = kClass ? k + nx : k;
   auto load = [&](int i, uint32_t (&wv)[NW]) {
     if (!kClass || i < k) {
       const int j = k - 1 - i;
-      const int y = 31 - __builtin_clz(rem);
-      rem ^= 1u << y;
-      const uint32_t c1 = s_cb[y * kCbStride + j + 1];
-      const uint32_t c0 = s_cb[y * kCbStride + j];
-      const uint32_t r2 = B - c1;
-      B = r2 + c0;
-      LoadTile<LB>(prev + (r2 << LB), wv);
+      const int z = __builtin_clz(rem);  // y_j = 31 - z
+      rem ^= 0x80000000u >> z;
+      const uint32_t c1 = s_lvl[z * kCbStride + j + 1];
+      const uint32_t c0 = s_lvl[z * kCbStride + j];
+      const uint32_t off = B - c1;  // rank(Hr \ y_j) << LB
+      B = off + c0;
+      LoadTile<LB>(prev + off, wv);
     } else {
       const int c = i - k;
       const uint8_t* base = c == 0 ? x[0] : c == 1 ? x[1] : c == 2 ? x[2] : x[3];
@@ -341,12 +359,19 @@ __device__ __forceinline__ void DPTile(const uint8_t* __restrict__ prev,
 }
 
 template <int LB>
+__device__ __forceinline__ void LoadPascal(const uint32_t* __restrict__ g_cb,
+                                           uint32_t* s_lvl) {
+  for (int i = threadIdx.x; i < 32 * kCbStride; i += blockDim.x)
+    s_lvl[i] = g_cb[kRevWord + i] << LB;
+  if (threadIdx.x <= kTailBits) s_lvl[kLvlOffs + threadIdx.x] = g_cb[kCbOffs + threadIdx.x];
+}
+
+template <int LB>
 __device__ __forceinline__ void LoadShared(const DPGraph* __restrict__ g,
                                            const uint32_t* __restrict__ g_cb,
-                                           uint32_t* s_cb, uint32_t* s_tab,
+                                           uint32_t* s_lvl, uint32_t* s_tab,
                                            bool load_cb) {
-  if (load_cb)
-    for (int i = threadIdx.x; i < kCbWords; i += blockDim.x) s_cb[i] = g_cb[i];
+  if (load_cb) LoadPascal<LB>(g_cb, s_lvl);
   // nbh, isoh and isol are consecutive in DPGraph
   for (int i = threadIdx.x; i < kTabWords; i += blockDim.x) s_tab[i] = g->nbh[i];
 }
@@ -360,7 +385,7 @@ __device__ __forceinline__ void DPLevelRange(const DPGraph* __restrict__ g,
                                              int h, uint32_t first,
                                              uint32_t end, uint32_t lane_id,
                                              uint32_t stride, int R,
-                                             const uint32_t* s_cb,
+                                             const uint32_t* s_lvl,
                                              const uint32_t* s_tab) {
   const int nh = static_cast<int>(g->nh);
   uint8_t* base = arena + g->table;
@@ -370,13 +395,13 @@ __device__ __forceinline__ void DPLevelRange(const DPGraph* __restrict__ g,
   const uint8_t* const none[4] = {nullptr, nullptr, nullptr, nullptr};
   const uint16_t* tail = reinterpret_cast<const uint16_t*>(g_cb + kTailWord);
   for (uint32_t t0 = first + lane_id * R; t0 < end; t0 += stride) {
-    uint32_t H = UnrankTiles(t0, h, nh, s_cb, tail);
+    uint32_t H = UnrankTiles<LB>(t0, h, nh, s_lvl, tail);
     const uint32_t tend = min(t0 + static_cast<uint32_t>(R), end);
     for (uint32_t t = t0;;) {
       if (h <= 1)
-        DPTile<LB, false, true>(prev, cur, t, H, h, H, h, none, 0, s_cb, s_tab);
+        DPTile<LB, false, true>(prev, cur, t, H, h, H, h, none, 0, s_lvl, s_tab);
       else
-        DPTile<LB, false, false>(prev, cur, t, H, h, H, h, none, 0, s_cb, s_tab);
+        DPTile<LB, false, false>(prev, cur, t, H, h, H, h, none, 0, s_lvl, s_tab);
       if (++t >= tend) break;
       H = NextCombination(H);
     }
@@ -394,16 +419,16 @@ template <int LB>
 __global__ __launch_bounds__(1024) void DPLevelKernel(
     const DPGraph* __restrict__ g, const uint32_t* __restrict__ g_cb,
     uint8_t* __restrict__ arena, int h, uint32_t ntiles, int R, int xcd_remap) {
-  __shared__ uint32_t s_cb[kCbWords];
+  __shared__ uint32_t s_lvl[kLvlWords];
   __shared__ uint32_t s_tab[kTabWords];
-  LoadShared<LB>(g, g_cb, s_cb, s_tab, true);
+  LoadShared<LB>(g, g_cb, s_lvl, s_tab, true);
   __syncthreads();
   uint32_t bid = blockIdx.x;
   if (xcd_remap && gridDim.x % kXcds == 0)
     bid = (bid % kXcds) * (gridDim.x / kXcds) + bid / kXcds;
   const uint32_t lane_id = bid * blockDim.x + threadIdx.x;
   const uint32_t stride = gridDim.x * blockDim.x * R;
-  DPLevelRange<LB>(g, g_cb, arena, h, 0, ntiles, lane_id, stride, R, s_cb, s_tab);
+  DPLevelRange<LB>(g, g_cb, arena, h, 0, ntiles, lane_id, stride, R, s_lvl, s_tab);
 }
 
 // Static schedule of one class-layout level launch: for each XCD, up to
@@ -431,9 +456,9 @@ template <int LB>
 __global__ __launch_bounds__(1024) void DPClassLevelKernel(
     const DPGraph* __restrict__ g, const uint32_t* __restrict__ g_cb,
     uint8_t* __restrict__ arena, int h, int R, const ClassSched sched) {
-  __shared__ uint32_t s_cb[kCbWords];
+  __shared__ uint32_t s_lvl[kLvlWords];
   __shared__ uint32_t s_tab[kTabWords];
-  LoadShared<LB>(g, g_cb, s_cb, s_tab, true);
+  LoadShared<LB>(g, g_cb, s_lvl, s_tab, true);
   __syncthreads();
   const int cb = static_cast<int>(g->cb);
   const int rest = static_cast<int>(g->nh) - cb;
@@ -457,14 +482,14 @@ __global__ __launch_bounds__(1024) void DPClassLevelKernel(
     for (uint32_t c0 = sg.first + wx * CH; c0 < sg.end; c0 += gx * CH) {
       const uint32_t t0 = c0 + threadIdx.x * static_cast<uint32_t>(R);
       if (t0 >= sg.end) continue;
-      uint32_t Hr = UnrankTiles(t0, k, rest, s_cb, tail);
+      uint32_t Hr = UnrankTiles<LB>(t0, k, rest, s_lvl, tail);
       const uint32_t tend = min(t0 + static_cast<uint32_t>(R), sg.end);
       for (uint32_t t = t0;;) {
         const uint32_t H = p | (Hr << cb);
         if (h <= 1)
-          DPTile<LB, true, true>(prev, cur, t, Hr, k, H, h, x, np, s_cb, s_tab);
+          DPTile<LB, true, true>(prev, cur, t, Hr, k, H, h, x, np, s_lvl, s_tab);
         else
-          DPTile<LB, true, false>(prev, cur, t, Hr, k, H, h, x, np, s_cb, s_tab);
+          DPTile<LB, true, false>(prev, cur, t, Hr, k, H, h, x, np, s_lvl, s_tab);
         if (++t >= tend) break;
         Hr = NextCombination(Hr);
       }
@@ -483,14 +508,14 @@ __global__ __launch_bounds__(1024) void DPChunkKernel(
     const DPGraph* __restrict__ gs, const uint32_t* __restrict__ g_cb,
     uint8_t* __restrict__ arena, const Chunk* __restrict__ chunks, int h,
     int R) {
-  __shared__ uint32_t s_cb[kCbWords];
+  __shared__ uint32_t s_lvl[kLvlWords];
   __shared__ uint32_t s_tab[kTabWords];
   const Chunk c = chunks[blockIdx.x];
   const DPGraph* g = gs + c.gi;
-  LoadShared<LB>(g, g_cb, s_cb, s_tab, true);
+  LoadShared<LB>(g, g_cb, s_lvl, s_tab, true);
   __syncthreads();
   DPLevelRange<LB>(g, g_cb, arena, h, c.first, c.end, threadIdx.x,
-                   blockDim.x * R, R, s_cb, s_tab);
+                   blockDim.x * R, R, s_lvl, s_tab);
 }
 
 // Persistent batch: workgroups pull whole graphs from a device queue (graphs
@@ -502,23 +527,23 @@ __global__ __launch_bounds__(1024) void DPPersistentKernel(
     const DPGraph* __restrict__ gs, const uint32_t* __restrict__ g_cb,
     uint8_t* __restrict__ arena, int ngraphs, uint32_t* __restrict__ queue,
     int R) {
-  __shared__ uint32_t s_cb[kCbWords];
+  __shared__ uint32_t s_lvl[kLvlWords];
   __shared__ uint32_t s_tab[kTabWords];
   __shared__ int s_gi;
-  for (int i = threadIdx.x; i < kCbWords; i += blockDim.x) s_cb[i] = g_cb[i];
+  LoadPascal<LB>(g_cb, s_lvl);
   for (;;) {
     if (threadIdx.x == 0) s_gi = static_cast<int>(atomicAdd(queue, 1u));
     __syncthreads();
     const int gi = s_gi;
     if (gi >= ngraphs) break;  // uniform: every thread read the same s_gi
     const DPGraph* g = gs + gi;
-    LoadShared<LB>(g, g_cb, s_cb, s_tab, false);
+    LoadShared<LB>(g, g_cb, s_lvl, s_tab, false);
     __syncthreads();
     const int nh = static_cast<int>(g->nh);
     for (int h = 0; h <= nh; ++h) {
-      const uint32_t ntiles = s_cb[nh * kCbStride + h];  // C(nh, h), no padding
+      const uint32_t ntiles = LevelTiles<LB>(s_lvl, nh, h);  // no padding
       DPLevelRange<LB>(g, g_cb, arena, h, 0, ntiles, threadIdx.x, blockDim.x * R, R,
-                       s_cb, s_tab);
+                       s_lvl, s_tab);
       __syncthreads();
     }
   }
@@ -531,14 +556,14 @@ template <int LB>
 __global__ __launch_bounds__(1024) void DPSmallLevelsKernel(
     const DPGraph* __restrict__ g, const uint32_t* __restrict__ g_cb,
     uint8_t* __restrict__ arena, int h0, int h1) {
-  __shared__ uint32_t s_cb[kCbWords];
+  __shared__ uint32_t s_lvl[kLvlWords];
   __shared__ uint32_t s_tab[kTabWords];
-  LoadShared<LB>(g, g_cb, s_cb, s_tab, true);
+  LoadShared<LB>(g, g_cb, s_lvl, s_tab, true);
   __syncthreads();
   const int nh = static_cast<int>(g->nh);
   for (int h = h0; h <= h1; ++h) {
-    const uint32_t ntiles = s_cb[nh * kCbStride + h];
-    DPLevelRange<LB>(g, g_cb, arena, h, 0, ntiles, threadIdx.x, blockDim.x, 1, s_cb, s_tab);
+    const uint32_t ntiles = LevelTiles<LB>(s_lvl, nh, h);
+    DPLevelRange<LB>(g, g_cb, arena, h, 0, ntiles, threadIdx.x, blockDim.x, 1, s_lvl, s_tab);
     __syncthreads();
   }
 }
@@ -774,11 +799,14 @@ const uint32_t* PascalDevice(td_ctx* ctx) {
   auto it = ctx->bufs.find("dp.pascal");
   if (it != ctx->bufs.end() && it->second.p)
     return static_cast<const uint32_t*>(it->second.p);
-  // binomials, O_k, then the 16-bit unrank table (2 entries per word)
-  std::vector<uint32_t> cb(kTailWord + (1u << kTailBits) / 2, 0);
+  // binomials, O_k, the 16-bit unrank table (2 entries per word), then the
+  // binomials by leading zeros of the vertex bit
+  std::vector<uint32_t> cb(kPascalWords, 0);
   for (int x = 0; x < 32; ++x)
-    for (int j = 0; j < kCbStride; ++j)
+    for (int j = 0; j < kCbStride; ++j) {
       cb[x * kCbStride + j] = static_cast<uint32_t>(Binom(x, j));
+      cb[kRevWord + (31 - x) * kCbStride + j] = static_cast<uint32_t>(Binom(x, j));
+    }
   uint16_t* tail = reinterpret_cast<uint16_t*>(cb.data() + kTailWord);
   uint32_t off = 0;
   for (int k = 0; k <= kTailBits; ++k) {
@@ -795,7 +823,7 @@ const uint32_t* PascalDevice(td_ctx* ctx) {
 int GridFor(td_ctx* ctx, uint64_t lanes_needed, int block) {
   const uint64_t blocks = (lanes_needed + block - 1) / block;
   const uint64_t cap = static_cast<uint64_t>(ctx->prop.multiProcessorCount) *
-                       (2048 / block) * 8;
+                       (2048 / block) * static_cast<uint64_t>(ctx->dp.grid_mult);
   return static_cast<int>(std::max<uint64_t>(1, std::min(blocks, cap)));
 }
 
@@ -1262,9 +1290,9 @@ std::string JsonLine(const std::string& key, const DPConfig& c, double ms, bool 
   std::snprintf(line, sizeof(line),
                 "{\"key\":\"%s\",\"best\":%s,\"low_bits\":%d,\"tiles_per_lane\":%d,"
                 "\"block\":%d,\"class_bits\":%d,\"batch_block\":%d,"
-                "\"batch_wg_per_cu\":%d,\"level_ms\":%.4f,\"arch\":\"%s\"}\n",
+                "\"batch_wg_per_cu\":%d,\"grid_mult\":%d,\"level_ms\":%.4f,\"arch\":\"%s\"}\n",
                 key.c_str(), best ? "true" : "false", c.low_bits, c.tiles_per_lane,
-                c.block, c.class_bits, c.batch_block, c.batch_wg_per_cu, ms, arch);
+                c.block, c.class_bits, c.batch_block, c.batch_wg_per_cu, c.grid_mult, ms, arch);
   return line;
 }
 
@@ -1285,6 +1313,8 @@ bool CacheLookup(const char* path, const std::string& key, const char* arch,
       continue;
     DPConfig c = *cfg;
     double m = 0;
+    (void)JsonInt(line, "grid_mult", &c.grid_mult);  // absent in older caches
+    if (c.grid_mult < 1 || c.grid_mult > 64) continue;
     if (JsonInt(line, "low_bits", &c.low_bits) && JsonInt(line, "tiles_per_lane", &c.tiles_per_lane) &&
         JsonInt(line, "block", &c.block) && JsonInt(line, "batch_block", &c.batch_block) &&
         JsonInt(line, "batch_wg_per_cu", &c.batch_wg_per_cu) && JsonDouble(line, "level_ms", &m) &&
@@ -1469,14 +1499,18 @@ int td_dp_autotune(td_ctx* ctx, int32_t n, const char* cache_path,
             adj[j] |= 1ull << i;
           }
       std::vector<DPConfig> cands;
+      // (tiles per lane, grid multiple): one tile per lane at three grid
+      // sizes, and two consecutive tiles per lane
+      const int shapes[4][2] = {{1, 2}, {1, 4}, {1, 8}, {2, 4}};
       for (int cbits : {0, 4})
         for (int lb : {3, 4, 5})
-          for (int r : {1, 2, 4, 8})
+          for (const auto& rg : shapes)
             for (int blk : {128, 256, 512}) {
               DPConfig c = ctx->dp;
               c.class_bits = cbits;
               c.low_bits = lb;
-              c.tiles_per_lane = r;
+              c.tiles_per_lane = rg[0];
+              c.grid_mult = rg[1];
               c.block = blk;
               c.use_graph = 0;
               cands.push_back(c);

@@ -40,8 +40,10 @@ int td_ctx_create(td_ctx** out, int device, uint64_t mem_budget_bytes);
 void td_ctx_destroy(td_ctx* ctx);
 /* Last error message of this context ("" if none). */
 const char* td_last_error(const td_ctx* ctx);
-/* Tuning knobs (see td_dp_autotune): "dp.low_bits" (2..5, default 5),
- * "dp.tiles_per_lane" (1..64, default 1), "dp.block" (64..1024, default 256),
+/* Tuning knobs (see td_dp_autotune): "dp.low_bits" (2..5, default 4),
+ * "dp.tiles_per_lane" (1..64, default 1), "dp.block" (64..1024, default 512),
+ * "dp.grid_mult" (1..64, default 4: level launches use at most this many
+ * times the workgroups the device holds at once),
  * "dp.batch_wg_per_cu", "dp.use_graph" (0/1), "dp.class_bits" (0, 3 or 4,
  * default 0: level tables grouped into 2^bits vertex classes, one class per
  * XCD, for single graphs with at least "dp.class_min_nh" (default 16)
@@ -115,7 +117,7 @@ int td_dp_solve_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
                       td_dp_stats* stats);
 
 /* Autotuner: times the DP level kernel for every candidate (class_bits,
- * low_bits, tiles_per_lane, block) on a seeded G(n, 0.25) graph and keeps the best in
+ * low_bits, tiles_per_lane, grid_mult, block) on a seeded G(n, 0.25) graph and keeps the best in
  * the context.  With a non-null cache_path (JSON lines), a "best" line for
  * this n and device arch is applied without timing anything; otherwise the
  * sweep runs and every candidate plus the best are appended.  best_ms_out

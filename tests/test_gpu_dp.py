@@ -117,7 +117,7 @@ def test_all_levels_bit_exact(ctx, oracle, low_bits, tiles):
                 got = dp.level(k)
                 assert np.array_equal(got, lv[off[k]: off[k + 1]]), (n, k)
     finally:
-        ctx.set_param("dp.low_bits", 5)
+        ctx.set_param("dp.low_bits", 4)
         ctx.set_param("dp.tiles_per_lane", 1)
 
 
@@ -169,6 +169,28 @@ def test_n24_vs_oracle(ctx, dp, oracle, graph, class_bits):
         ctx.set_param("dp.class_bits", 0)
 
 
+@pytest.mark.parametrize("low_bits,grid_mult", [(2, 1), (3, 2), (4, 1), (5, 1)])
+def test_n22_all_levels_grid_stride(ctx, oracle, low_bits, grid_mult):
+    """n = 22, every level against the oracle: more than 16 vertices above
+    the low bits (the unrank scan runs before the 16-vertex table), and a
+    small grid so that lanes walk several tiles each."""
+    g = G.gnp(22, 0.22, 400 + low_bits)
+    print("n22", low_bits, grid_mult, flush=True)
+    td, par, lv = oracle.dp_fast(g, levels=True)
+    ctx.set_param("dp.low_bits", low_bits)
+    ctx.set_param("dp.grid_mult", grid_mult)
+    try:
+        dp = api.DynamicGPU(ctx)
+        dp(g)
+        assert (dp.treedepth, dp.parent) == (td, par)
+        off = level_offsets(g.n)
+        for k in range(g.n + 1):
+            assert np.array_equal(dp.level(k), lv[off[k]: off[k + 1]]), k
+    finally:
+        ctx.set_param("dp.low_bits", 4)
+        ctx.set_param("dp.grid_mult", 4)
+
+
 @pytest.mark.parametrize("class_bits,low_bits,tiles", [(3, 5, 1), (4, 5, 1), (4, 2, 3),
                                                        (4, 4, 4), (3, 3, 2)])
 def test_class_layout_all_levels_bit_exact(ctx, oracle, class_bits, low_bits, tiles):
@@ -202,7 +224,7 @@ def test_class_layout_all_levels_bit_exact(ctx, oracle, class_bits, low_bits, ti
     finally:
         ctx.set_param("dp.class_bits", 0)
         ctx.set_param("dp.class_min_nh", 16)
-        ctx.set_param("dp.low_bits", 5)
+        ctx.set_param("dp.low_bits", 4)
         ctx.set_param("dp.tiles_per_lane", 1)
 
 
@@ -317,7 +339,7 @@ def test_autotune_single_and_cache(ctx, oracle, tmp_path):
     without a sweep (the file does not grow); results stay bit-exact."""
     import json
 
-    keys = ("dp.low_bits", "dp.tiles_per_lane", "dp.block", "dp.class_bits")
+    keys = ("dp.low_bits", "dp.tiles_per_lane", "dp.block", "dp.class_bits", "dp.grid_mult")
     saved = {k: ctx.get_param(k) for k in keys}
     cache = tmp_path / "tune.jsonl"
     try:
@@ -330,7 +352,8 @@ def test_autotune_single_and_cache(ctx, oracle, tmp_path):
         assert best == pytest.approx(win[0]["level_ms"], abs=1e-4)
         got = {k: ctx.get_param(k) for k in keys}
         assert got == {"dp.low_bits": win[0]["low_bits"], "dp.tiles_per_lane": win[0]["tiles_per_lane"],
-                       "dp.block": win[0]["block"], "dp.class_bits": win[0]["class_bits"]}
+                       "dp.block": win[0]["block"], "dp.class_bits": win[0]["class_bits"],
+                       "dp.grid_mult": win[0]["grid_mult"]}
         size = cache.stat().st_size
         for k, v in saved.items():
             ctx.set_param(k, v)
