@@ -24,6 +24,7 @@
 #include <cstring>
 #include <fstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ctx.hpp"
@@ -725,6 +726,32 @@ __global__ __launch_bounds__(256) void DPExportLevelKernel(
 constexpr int kMaxDPN = 32;
 constexpr int kMaxBatchN = 26;
 
+// Masks over the low parts L of a tile (tile-mask bit order: bit
+// 8 (L & 3) + (L >> 2)) that depend only on lb: all L, the L meeting a low
+// set T, and the L without low vertex y.
+struct LowMasks {
+  uint32_t all = 0, meet[32] = {}, without[5] = {};
+};
+const LowMasks& LowMasksFor(int lb) {
+  static const auto tables = [] {
+    std::vector<LowMasks> t(6);
+    for (int b = 0; b <= 5; ++b) {
+      LowMasks& M = t[b];
+      const int NB = 1 << b;
+      auto bit = [](int L) { return 1u << (8 * (L & 3) + (L >> 2)); };
+      for (int L = 0; L < NB; ++L) M.all |= bit(L);
+      for (int T = 1; T < NB; ++T)
+        for (int L = 0; L < NB; ++L)
+          if (L & T) M.meet[T] |= bit(L);
+      for (int y = 0; y < b; ++y)
+        for (int L = 0; L < NB; ++L)
+          if (!(L >> y & 1)) M.without[y] |= bit(L);
+    }
+    return t;
+  }();
+  return tables[lb];
+}
+
 // Tile offsets of each high level, each level padded to a 256-byte boundary
 // so no cache line holds tiles of two levels.
 DPGraph MakeGraph(int n, const uint64_t* adj, int lb, uint64_t table_off, int cb = 0) {
@@ -760,33 +787,23 @@ DPGraph MakeGraph(int n, const uint64_t* adj, int lb, uint64_t table_off, int cb
       G.nbh[128 * c + chunk] = G.nbh[128 * c + (chunk & (chunk - 1))] | row;
     }
   }
-  // isolation masks over the low parts L (tile-mask bit order)
-  const int NB = 1 << lb;
-  auto bit = [](int L) { return 1u << (8 * (L & 3) + (L >> 2)); };
+  // isolation masks over the low parts L (tile-mask bit order), each built
+  // from the one with its lowest element removed
+  const LowMasks& M = LowMasksFor(lb);
   const uint32_t lowmask = (1u << lb) - 1u;
   for (int c = 0; c < 4; ++c) {
-    uint32_t all = 0;
-    for (int L = 0; L < NB; ++L) all |= bit(L);
-    G.isoh[128 * c] = all;
+    G.isoh[128 * c] = M.all;
     for (int chunk = 1; chunk < 128; ++chunk) {
-      const int b = __builtin_ctz(static_cast<unsigned>(chunk));
-      const int x = 7 * c + b;
+      const int x = 7 * c + __builtin_ctz(static_cast<unsigned>(chunk));
       const uint32_t low = x < static_cast<int>(G.nh) ? G.adj[lb + x] & lowmask : 0u;
-      uint32_t reach = 0;  // the L meeting N(x)
-      for (int L = 0; L < NB; ++L)
-        if (static_cast<uint32_t>(L) & low) reach |= bit(L);
-      G.isoh[128 * c + chunk] = G.isoh[128 * c + (chunk & (chunk - 1))] & reach;
+      G.isoh[128 * c + chunk] = G.isoh[128 * c + (chunk & (chunk - 1))] & M.meet[low];
     }
   }
-  for (int Y = 0; Y < NB; ++Y) {
-    uint32_t m = 0;
-    for (int L = 0; L < NB; ++L) {
-      bool ok = true;
-      for (int y = 0; y < lb; ++y)
-        if ((Y & L) >> y & 1 && (G.adj[y] & static_cast<uint32_t>(L)) == 0u) ok = false;
-      if (ok) m |= bit(L);
-    }
-    G.isol[Y] = m;
+  // L is fine for low vertex y iff y is not in L or L meets N(y)
+  G.isol[0] = M.all;
+  for (int Y = 1; Y < (1 << lb); ++Y) {
+    const int y = __builtin_ctz(static_cast<unsigned>(Y));
+    G.isol[Y] = G.isol[Y & (Y - 1)] & (M.without[y] | M.meet[G.adj[y] & lowmask]);
   }
   return G;
 }
@@ -1092,7 +1109,7 @@ void SolveOne(td_ctx* ctx, int n, const uint64_t* adj, uint8_t* td_out,
 // ---------------------------------------------------------------------------
 template <int LB>
 void LaunchBatch(td_ctx* ctx, int mode, int ngraphs,
-                 const std::vector<DPGraph>& Gs, const DPGraph* dGs,
+                 const DPGraph* Gs, const DPGraph* dGs,
                  const uint32_t* cb, uint8_t* arena, uint32_t* dqueue,
                  uint64_t* launches) {
   const int R = ctx->dp.tiles_per_lane, block = ctx->dp.block;
@@ -1135,7 +1152,7 @@ void LaunchBatch(td_ctx* ctx, int mode, int ngraphs,
     }
   } else if (mode == 1) {
     int maxnh = 0;
-    for (auto& G : Gs) maxnh = std::max(maxnh, static_cast<int>(G.nh));
+    for (int i = 0; i < ngraphs; ++i) maxnh = std::max(maxnh, static_cast<int>(Gs[i].nh));
     const uint32_t chunk_tiles = static_cast<uint32_t>(block) * R * 2;
     std::vector<std::vector<Chunk>> per_level(maxnh + 1);
     size_t total = 0;
@@ -1200,11 +1217,27 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
   for (int i = 0; i < ngraphs; ++i) perm[i] = i;
   std::stable_sort(perm.begin(), perm.end(),
                    [&](int a, int b) { return ns[a] > ns[b]; });
-  std::vector<DPGraph> Gs(ngraphs);
+  // the graph records go up from page-locked memory; they are built on
+  // several host threads (about 1 us each), then given their arena offsets
+  DPGraph* Gs = static_cast<DPGraph*>(ctx->Pinned(sizeof(DPGraph) * ngraphs));
+  {
+    const int nthreads = static_cast<int>(std::min<unsigned>(
+        16u, std::max(1u, std::min(std::thread::hardware_concurrency(),
+                                   static_cast<unsigned>(ngraphs / 128)))));
+    auto work = [&](int t) {
+      for (int j = t; j < ngraphs; j += nthreads) {
+        const int i = perm[j];
+        Gs[j] = MakeGraph(ns[i], adj_concat + row_off[i], lb, 0);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+  }
   uint64_t off = 0;
   for (int j = 0; j < ngraphs; ++j) {
-    const int i = perm[j];
-    Gs[j] = MakeGraph(ns[i], adj_concat + row_off[i], lb, off);
+    Gs[j].table = off;
     off += (TableBytes(Gs[j]) + 255) / 256 * 256;
   }
   uint8_t* arena = static_cast<uint8_t*>(ctx->Buffer("dp.batch_arena", off));
@@ -1218,7 +1251,7 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
   std::vector<uint32_t> start(ngraphs);
   for (int j = 0; j < ngraphs; ++j)
     start[j] = Gs[j].n >= 32 ? 0xFFFFFFFFu : ((1u << Gs[j].n) - 1u);
-  TD_HIP(hipMemcpyAsync(dGs, Gs.data(), sizeof(DPGraph) * ngraphs,
+  TD_HIP(hipMemcpyAsync(dGs, Gs, sizeof(DPGraph) * ngraphs,
                         hipMemcpyHostToDevice, ctx->stream));
   TD_HIP(hipMemcpyAsync(dstart, start.data(), 4 * ngraphs, hipMemcpyHostToDevice, ctx->stream));
   uint64_t launches = 0;
