@@ -574,15 +574,38 @@ __global__ void ResetCounter(uint32_t* c) {
 }
 
 // td of subset S (raw byte >> 1) of graph g.
+// Binomials and, for a class layout, the first tile of each class per high
+// level: s_cbase[16 h + p] = ClassBase(h, p, nh - cb).  Ends with a barrier.
+constexpr int kCbaseWords = 33 * 16;
+__device__ __forceinline__ void LoadLookupTables(const DPGraph* __restrict__ g,
+                                                 const uint32_t* __restrict__ g_cb,
+                                                 uint32_t* s_cb, uint32_t* s_cbase) {
+  for (int i = threadIdx.x; i < 32 * kCbStride; i += blockDim.x) s_cb[i] = g_cb[i];
+  __syncthreads();
+  const int cb = static_cast<int>(g->cb), nh = static_cast<int>(g->nh);
+  if (cb > 0)
+    for (int i = threadIdx.x; i < (nh + 1) * 16; i += blockDim.x)
+      s_cbase[i] = ClassBase(i >> 4, static_cast<uint32_t>(i & 15), nh - cb,
+                             [&](int x, int j) { return s_cb[x * kCbStride + j]; });
+  __syncthreads();
+}
+
+// td of subset S (bit v = vertex v) from the finished tables.
 __device__ __forceinline__ uint32_t Lookup(const DPGraph* __restrict__ g,
                                            const uint8_t* __restrict__ arena,
-                                           const uint32_t* s_cb, uint32_t S) {
+                                           const uint32_t* s_cb, const uint32_t* s_cbase,
+                                           uint32_t S) {
   const uint32_t lb = g->lb;
+  const int cb = static_cast<int>(g->cb);
   const uint32_t L = S & ((1u << lb) - 1u);
   const uint32_t H = S >> lb;
   const int h = __builtin_popcount(H);
-  const uint32_t r = TileIndex(H, static_cast<int>(g->nh), static_cast<int>(g->cb),
-                               [&](int x, int j) { return s_cb[x * kCbStride + j]; });
+  uint32_t r = cb > 0 ? s_cbase[16 * h + (H & ((1u << cb) - 1u))] : 0u;
+  uint32_t Hr = H >> cb;
+  for (int i = 1; Hr; ++i) {
+    r += s_cb[__builtin_ctz(Hr) * kCbStride + i];
+    Hr &= Hr - 1;
+  }
   const size_t addr =
       g->table + ((static_cast<size_t>(g->lvl[h]) + r) << lb) + L;
   return static_cast<uint32_t>(arena[addr]) >> 1;
@@ -605,10 +628,9 @@ __global__ __launch_bounds__(64) void DPWalkKernel(
     const int32_t* __restrict__ nsteps, int32_t* __restrict__ orders,
     int order_stride) {
   __shared__ uint32_t s_cb[32 * kCbStride];
-  for (int i = threadIdx.x; i < 32 * kCbStride; i += blockDim.x)
-    s_cb[i] = g_cb[i];
-  __syncthreads();
+  __shared__ uint32_t s_cbase[kCbaseWords];
   const DPGraph* g = gs + blockIdx.x;
+  LoadLookupTables(g, g_cb, s_cb, s_cbase);
   const int lane = threadIdx.x;
   const uint32_t myadj = lane < 32 ? g->adj[lane] : 0u;
   uint32_t S = start[blockIdx.x];
@@ -631,8 +653,8 @@ __global__ __launch_bounds__(64) void DPWalkKernel(
       rest &= ~comp;
     }
     // one round of lookups: td of v's component and of that component minus v
-    const uint32_t tdc = in ? Lookup(g, arena, s_cb, mine) : 0u;
-    const uint32_t tv = in ? Lookup(g, arena, s_cb, mine & ~(1u << lane)) : 0xFFu;
+    const uint32_t tdc = in ? Lookup(g, arena, s_cb, s_cbase, mine) : 0u;
+    const uint32_t tv = in ? Lookup(g, arena, s_cb, s_cbase, mine & ~(1u << lane)) : 0xFFu;
     const uint32_t T = WaveMax(tdc);
     // the first component (lowest vertex order) of largest td holds the
     // lowest vertex whose component has td T
@@ -657,10 +679,9 @@ __global__ __launch_bounds__(64) void DPTreeKernel(
     const uint8_t* __restrict__ arena, const uint32_t* __restrict__ start,
     int8_t* __restrict__ parent_out, int32_t* __restrict__ td_out) {
   __shared__ uint32_t s_cb[32 * kCbStride];
-  for (int i = threadIdx.x; i < 32 * kCbStride; i += blockDim.x)
-    s_cb[i] = g_cb[i];
-  __syncthreads();
+  __shared__ uint32_t s_cbase[kCbaseWords];
   const DPGraph* g = gs + blockIdx.x;
+  LoadLookupTables(g, g_cb, s_cb, s_cbase);
   const int lane = threadIdx.x;
   const uint32_t myadj = lane < 32 ? g->adj[lane] : 0u;
   uint32_t S = start[blockIdx.x];
@@ -681,8 +702,8 @@ __global__ __launch_bounds__(64) void DPTreeKernel(
       if ((comp >> lane) & 1u) mine = comp;
       rest &= ~comp;
     }
-    const uint32_t tdc = in ? Lookup(g, arena, s_cb, mine) : 0u;
-    const uint32_t tv = in ? Lookup(g, arena, s_cb, mine & ~(1u << lane)) : 0xFFu;
+    const uint32_t tdc = in ? Lookup(g, arena, s_cb, s_cbase, mine) : 0u;
+    const uint32_t tv = in ? Lookup(g, arena, s_cb, s_cbase, mine & ~(1u << lane)) : 0xFFu;
     const uint32_t cand = static_cast<uint32_t>(__ballot(in && tv + 1u == tdc));
     // a connected set always has a vertex with td(C \ v) = td(C) - 1; the
     // lowest vertex stands in only so that a damaged table cannot stall
@@ -709,14 +730,13 @@ __global__ __launch_bounds__(256) void DPExportLevelKernel(
     const uint8_t* __restrict__ arena, int k, uint32_t count,
     uint8_t* __restrict__ out) {
   __shared__ uint32_t s_cb[32 * kCbStride];
-  for (int i = threadIdx.x; i < 32 * kCbStride; i += blockDim.x)
-    s_cb[i] = g_cb[i];
-  __syncthreads();
+  __shared__ uint32_t s_cbase[kCbaseWords];
+  LoadLookupTables(g, g_cb, s_cb, s_cbase);
   const int n = static_cast<int>(g->n);
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < count;
        r += gridDim.x * blockDim.x) {
     const uint32_t S = Unrank(r, k, n, s_cb);
-    out[r] = static_cast<uint8_t>(Lookup(g, arena, s_cb, S));
+    out[r] = static_cast<uint8_t>(Lookup(g, arena, s_cb, s_cbase, S));
   }
 }
 
