@@ -103,6 +103,14 @@ def load_library():
         "td_ctx_copy_bench": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, P(C.c_double)]),
         "td_heuristic_batch": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
                                          C.c_int32, P(C.c_int32), P(C.c_int8)]),
+        "td_bottom_up": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_uint64), C.c_int32,
+                                   P(C.c_int32), P(C.c_int8), P(C.c_int32)]),
+        "td_bottom_up_get_treedepth": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64,
+                                                 P(C.c_uint32)]),
+        "td_bottom_up_get_elimination": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64,
+                                                   P(C.c_int32)]),
+        "td_bottom_up_get_level": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int8), P(C.c_uint8),
+                                             C.c_uint64]),
     }
     sig.update({
         "td_bnb_solve_batch": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_uint64),
@@ -279,6 +287,73 @@ class DynamicGPU:
         out = np.zeros(max(comb(self.n, k), 1), np.uint8)
         self.ctx.check(self.lib.td_dp_get_level(self.ctx.h, k, _ptr(out, C.c_uint8), out.size))
         return out[: comb(self.n, k)]
+
+
+class BottomUpHeuristicGPU:
+    """td::BottomUpHeuristicGPU (bottom_up_heuristic.hpp:11-35) with the
+    queries of BottomUpHeuristicGPUAlgorithm (bottom_up_heuristic_gpu_algorithm.hpp).
+
+    ``bu(g)`` runs the bottom-up DP (all levels memory allows, or
+    ``iterations`` levels); ``treedepth`` is -1 when not every level ran
+    (the reference's UINT_MAX), else the depth of ``parent``'s forest.
+    """
+
+    def __init__(self, ctx: Optional[Context] = None):
+        self.ctx = ctx or Context()
+        self.lib = self.ctx.lib
+        self.n = 0
+        self.treedepth = None
+        self.parent = None
+        self.iterations = 0
+
+    def __call__(self, g: Graph, iterations: int = -1) -> "BottomUpHeuristicGPU":
+        adj = g.adj_array()
+        td = C.c_int32()
+        it = C.c_int32()
+        par = np.zeros(max(g.n, 1), np.int8)
+        self.ctx.check(self.lib.td_bottom_up(self.ctx.h, g.n, _ptr(adj, C.c_uint64), iterations,
+                                             C.byref(td), _ptr(par, C.c_int8), C.byref(it)))
+        self.n = g.n
+        self.treedepth = td.value
+        self.iterations = it.value
+        self.parent = par[: g.n].astype(int).tolist() if td.value >= 0 else None
+        return self
+
+    def Get(self, g: Graph) -> Result:
+        self(g)
+        if self.treedepth < 0:
+            return Result([], 2 ** 32 - 1, 0, [], [])
+        return Result.from_parent(self.parent, self.treedepth)
+
+    def GetIterationsPerformed(self) -> int:
+        return self.iterations
+
+    def GetTreedepth(self, nverts: int, subset_size: int, subset_code: int) -> int:
+        if nverts != self.n:
+            raise TDError(TD_E_INVALID, "nverts differs from the solved graph")
+        v = C.c_uint32()
+        self.ctx.check(self.lib.td_bottom_up_get_treedepth(self.ctx.h, subset_size, subset_code,
+                                                           C.byref(v)))
+        return v.value
+
+    def level(self, k: int):
+        """(last vertex, height) arrays of level k in colex order."""
+        from math import comb
+
+        c = comb(self.n, k)
+        last = np.zeros(max(c, 1), np.int8)
+        height = np.zeros(max(c, 1), np.uint8)
+        self.ctx.check(self.lib.td_bottom_up_get_level(self.ctx.h, k, _ptr(last, C.c_int8),
+                                                       _ptr(height, C.c_uint8), c))
+        return last[:c], height[:c]
+
+    def GetElimination(self, nverts: int, subset_size: int, subset_code: int) -> List[int]:
+        if nverts != self.n:
+            raise TDError(TD_E_INVALID, "nverts differs from the solved graph")
+        out = np.zeros(max(subset_size, 1), np.int32)
+        self.ctx.check(self.lib.td_bottom_up_get_elimination(self.ctx.h, subset_size,
+                                                             subset_code, _ptr(out, C.c_int32)))
+        return out[:subset_size].tolist()
 
 
 def pack_graphs(graphs: Sequence[Graph]):

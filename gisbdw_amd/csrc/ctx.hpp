@@ -149,6 +149,12 @@ struct td_ctx {
   bool have_dp = false;
   tdhip::DPGraph last_graph{};
 
+  // state of the last td_bottom_up (for its queries): vertex count, levels
+  // run, and the offset of each level in the history arrays
+  int bu_n = 0, bu_iters = 0;
+  bool have_bu = false;
+  std::vector<uint64_t> bu_off;
+
   // cached hipGraph per (n, config) for the single-graph level sequence
   struct GraphKey {
     int n, lb, r, block;

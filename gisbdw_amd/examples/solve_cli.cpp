@@ -1,9 +1,10 @@
 // solve_cli.cpp -- the reference CLI's dynGPU / bnbGPU paths and its
-// highestDegreeHeur / spanningTreeHeur / varianceHeur options
-// (Treedepth/main.cpp:178-202) on libtdhip.so through include/tdhip.hpp.
+// highestDegreeHeur / spanningTreeHeur / varianceHeur / bottomUpHeur options
+// (Treedepth/main.cpp:178-205) on libtdhip.so through include/tdhip.hpp.
 // Reads the reference's graph text ("N;" vertex lines, "a -- b" edges) and
 // prints treedepth, root and the decomposition's edges.
-//   solve_cli -a dynGPU|bnbGPU|highestDegreeHeur|spanningTreeHeur|varianceHeur <graph file>
+//   solve_cli -a dynGPU|bnbGPU|highestDegreeHeur|spanningTreeHeur|varianceHeur|bottomUpHeur
+//             <graph file>
 #include <fstream>
 #include <iostream>
 #include <regex>
@@ -15,7 +16,7 @@
 int main(int argc, char** argv) {
   if (argc != 4 || std::string(argv[1]) != "-a") {
     std::cerr << "usage: solve_cli -a dynGPU|bnbGPU|highestDegreeHeur|spanningTreeHeur|"
-                 "varianceHeur <graph file>\n";
+                 "varianceHeur|bottomUpHeur <graph file>\n";
     return 2;
   }
   const std::string alg = argv[2];
@@ -54,6 +55,9 @@ int main(int argc, char** argv) {
                                                     : td::HeuristicGPU::kVariance;
       r = td::HeuristicGPU(kind, ctx).Get(g);
       certified = false;  // an upper bound
+    } else if (alg == "bottomUpHeur") {
+      r = td::BottomUpHeuristicGPU(ctx).Get(g);
+      certified = false;  // an upper bound (UINT_MAX if the levels did not fit)
     } else {
       std::cerr << "Wrong algorithm option specified.\n";
       return 2;

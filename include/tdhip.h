@@ -152,6 +152,38 @@ int td_heuristic_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
                        const uint64_t* adj_concat, int32_t kind, int32_t* td_out,
                        int8_t* parent_concat);
 
+/* BottomUpHeuristicGPU::Get (Treedepth/src/branch_and_bound/heuristics/
+ * bottom_up_heuristic.hpp:17-31) over BottomUpHeuristicGPUAlgorithm::
+ * operator()(g, k) (bottom_up_heuristic_gpu_algorithm.hpp:67-90): levels
+ * 0..k-1 of the bottom-up union-find DP for a graph of n <= 32 vertices,
+ * with k = iterations, or with iterations = -1 as many levels as memory
+ * holds (GetMaxIterations, bottom_up_heuristic_gpu_algorithm.cu:152-175; here
+ * the context budget or the device's free memory).  *iterations_out
+ * (nullable) = levels run (GetIterationsPerformed).  With all n + 1 levels,
+ * *treedepth_out = depth of the elimination forest of GetElimination(n, n, 0)
+ * and parent_out (nullable, n entries, -1 = root) that forest; otherwise
+ * *treedepth_out = -1 (the reference's UINT_MAX).  An upper bound. */
+int td_bottom_up(td_ctx* ctx, int32_t n, const uint64_t* adj, int32_t iterations,
+                 int32_t* treedepth_out, int8_t* parent_out, int32_t* iterations_out);
+/* BottomUpHeuristicGPUAlgorithm::GetTreedepth(nverts, subset_size,
+ * subset_code) (bottom_up_heuristic_gpu_algorithm.cu:181-188) over the last
+ * td_bottom_up: the kept forest's height for that subset (0 for the empty
+ * set, and 0 for a size whose level was not run, like the reference). */
+int td_bottom_up_get_treedepth(td_ctx* ctx, int32_t subset_size, uint64_t subset_code,
+                               uint32_t* td_out);
+/* Level k of the last td_bottom_up's history in colex order (history_,
+ * bottom_up_heuristic_gpu_algorithm.hpp:60-61): the vertex added last and
+ * the height of every k-subset (nullable outputs, out_len >= C(n, k)). */
+int td_bottom_up_get_level(td_ctx* ctx, int32_t k, int8_t* last_out, uint8_t* height_out,
+                           uint64_t out_len);
+/* BottomUpHeuristicGPUAlgorithm::GetElimination(nverts, subset_size,
+ * subset_code) (bottom_up_heuristic_gpu_algorithm.hpp:92-108): subset_size
+ * vertices into order_out.  A size whose level was not run is TD_E_STATE
+ * (the reference returns an empty list, or reads past its history at
+ * subset_size == levels run). */
+int td_bottom_up_get_elimination(td_ctx* ctx, int32_t subset_size, uint64_t subset_code,
+                                 int32_t* order_out);
+
 /* ---------------------------------------------------------------------- */
 /* Branch-and-bound                                                        */
 /* ---------------------------------------------------------------------- */

@@ -11,6 +11,7 @@
 #define TDHIP_HPP_
 
 #include <cstdint>
+#include <limits>
 #include <list>
 #include <memory>
 #include <stdexcept>
@@ -188,6 +189,72 @@ class HeuristicGPU {
  private:
   Kind kind_;
   std::shared_ptr<Context> ctx_;
+};
+
+// td::BottomUpHeuristicGPUAlgorithm (Treedepth/src/branch_and_bound/
+// heuristics/bottom_up_heuristic_gpu_algorithm.hpp:15-65): operator()(g, k)
+// runs k levels (default: as many as memory holds), then the queries.
+class BottomUpHeuristicGPUAlgorithm {
+ public:
+  explicit BottomUpHeuristicGPUAlgorithm(std::shared_ptr<Context> ctx = std::make_shared<Context>())
+      : ctx_(std::move(ctx)) {}
+
+  void operator()(const Graph& g, int iterations = -1) {
+    parent_.assign(static_cast<std::size_t>(g.n), 0);
+    int32_t it = 0;
+    ctx_->Check(td_bottom_up(ctx_->get(), g.n, g.adj.data(), iterations, &treedepth_,
+                             parent_.data(), &it));
+    iterations_ = static_cast<std::size_t>(it);
+  }
+  std::size_t GetIterationsPerformed() const { return iterations_; }
+  unsigned GetTreedepth(std::size_t /*nverts*/, std::size_t subset_size,
+                        std::size_t subset_code) const {
+    uint32_t v = 0;
+    ctx_->Check(td_bottom_up_get_treedepth(ctx_->get(), static_cast<int32_t>(subset_size),
+                                           subset_code, &v));
+    return v;
+  }
+  template <typename VertexType = std::size_t>
+  std::list<VertexType> GetElimination(std::size_t /*nverts*/, std::size_t subset_size,
+                                       std::size_t subset_code) const {
+    std::vector<int32_t> order(subset_size ? subset_size : 1);
+    ctx_->Check(td_bottom_up_get_elimination(ctx_->get(), static_cast<int32_t>(subset_size),
+                                             subset_code, order.data()));
+    std::list<VertexType> out;
+    for (std::size_t i = 0; i < subset_size; ++i) out.push_back(static_cast<VertexType>(order[i]));
+    return out;
+  }
+  // the decomposition td_bottom_up built (valid after a full run)
+  int32_t Treedepth() const { return treedepth_; }
+  const std::vector<int8_t>& Parent() const { return parent_; }
+
+ private:
+  std::shared_ptr<Context> ctx_;
+  int32_t treedepth_ = -1;
+  std::vector<int8_t> parent_;
+  std::size_t iterations_ = 0;
+};
+
+// td::BottomUpHeuristicGPU::Get (bottom_up_heuristic.hpp:17-31): the
+// decomposition, or treedepth UINT_MAX when not every level fitted.
+class BottomUpHeuristicGPU {
+ public:
+  explicit BottomUpHeuristicGPU(std::shared_ptr<Context> ctx = std::make_shared<Context>())
+      : algo_(std::move(ctx)) {}
+  Result Get(const Graph& g) {
+    algo_(g);
+    if (algo_.Treedepth() < 0) {
+      Result r;
+      r.treedepth = std::numeric_limits<unsigned>::max();
+      return r;
+    }
+    const auto& p = algo_.Parent();
+    return Result::FromParent(std::vector<int>(p.begin(), p.end()),
+                              static_cast<unsigned>(algo_.Treedepth()));
+  }
+
+ private:
+  BottomUpHeuristicGPUAlgorithm algo_;
 };
 
 }  // namespace td

@@ -98,6 +98,32 @@ int32_t or_heuristic(int32_t n, const uint64_t* adj, int32_t heur_chain,
 /* EdgeLowerBound as the reference computes it (double sqrt/ceil). */
 uint32_t or_edge_lb_double(int32_t n, int32_t e);
 
+/* ---- BottomUpHeuristicGPU (Treedepth/src/branch_and_bound/heuristics/
+ *      bottom_up_heuristic.hpp, bottom_up_heuristic_gpu_algorithm.{hpp,cu};
+ *      Treedepth/src/union_find/ext_array_union_find.hpp) -------------- */
+/* Levels 0..iterations-1 (iterations <= n+1) of the bottom-up union-find DP.
+ * last_out / height_out (nullable, 2^n bytes; level k at offset
+ * sum_{j<k} C(n,j), colex order): the vertex added last and the forest
+ * height of each subset, as history_ holds them ((-1, 0) for the empty set).
+ * With iterations == n+1 returns the height of the full set, writes
+ * GetElimination(n, n, 0) into order (nullable, n entries) and the
+ * EliminationTree decomposition of that order into parent/root (nullable);
+ * otherwise returns -1 (the UINT_MAX of BottomUpHeuristicGPU::Get).
+ * Pinned by: a second, independent formulation (or_bottom_up_components)
+ * agreeing on every subset; height >= exact td on every subset; the order's
+ * decomposition passes the certifier with depth equal to the stored height;
+ * closed forms (complete, edgeless, star graphs).  The exact choice of the
+ * last vertex (first of strictly smallest height, ascending) follows the
+ * reference's code; no reference output pins it ("parity partially pinned").
+ */
+int32_t or_bottom_up(int32_t n, const uint64_t* adj, int32_t iterations,
+                     int8_t* last_out, uint8_t* height_out, int32_t* order,
+                     int32_t* parent, int32_t* root);
+/* The same heuristic on component heights instead of union-find arrays
+ * (independent code, for cross-checking or_bottom_up). */
+int32_t or_bottom_up_components(int32_t n, const uint64_t* adj, int32_t iterations,
+                                int8_t* last_out, uint8_t* height_out);
+
 /* ---- exact memoised solver (checker for n in [40,64]) ------------------ */
 /* Top-down decision search with a component memo: exact treedepth for
  * graphs too large for the subset DP.  Independent code from the GPU B&B.

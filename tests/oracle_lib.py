@@ -47,6 +47,10 @@ class Oracle:
             "or_edge_lb_double": (C.c_uint32, [C.c_int32, C.c_int32]),
             "or_exact_memo": (C.c_int32, [C.c_int32, P(C.c_uint64), C.c_uint64, P(C.c_int8),
                                           P(C.c_uint64)]),
+            "or_bottom_up": (C.c_int32, [C.c_int32, P(C.c_uint64), C.c_int32, P(C.c_int8),
+                                         P(C.c_uint8), P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
+            "or_bottom_up_components": (C.c_int32, [C.c_int32, P(C.c_uint64), C.c_int32,
+                                                    P(C.c_int8), P(C.c_uint8)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -173,3 +177,30 @@ class Oracle:
         td = self.L.or_exact_memo(g.n, _p(adj, C.c_uint64), node_budget, _p(par, C.c_int8),
                                   C.byref(nodes))
         return td, par.astype(int).tolist(), nodes.value
+
+    # ---- bottom-up heuristic ----------------------------------------------
+    def bottom_up(self, g, iterations=None, history=True):
+        """(td or -1, order, parent, root, last[2^n] or None, height[2^n] or None)."""
+        n = g.n
+        it = n + 1 if iterations is None else iterations
+        adj = g.adj_array()
+        last = np.zeros(1 << n, np.int8) if history else None
+        height = np.zeros(1 << n, np.uint8) if history else None
+        order = np.zeros(n, np.int32)
+        par = np.zeros(n, np.int32)
+        root = C.c_int32()
+        td = self.L.or_bottom_up(n, _p(adj, C.c_uint64), it,
+                                 _p(last, C.c_int8) if history else None,
+                                 _p(height, C.c_uint8) if history else None,
+                                 _p(order, C.c_int32), _p(par, C.c_int32), C.byref(root))
+        return td, order.tolist(), par.tolist(), root.value, last, height
+
+    def bottom_up_components(self, g, iterations=None):
+        n = g.n
+        it = n + 1 if iterations is None else iterations
+        adj = g.adj_array()
+        last = np.zeros(1 << n, np.int8)
+        height = np.zeros(1 << n, np.uint8)
+        td = self.L.or_bottom_up_components(n, _p(adj, C.c_uint64), it, _p(last, C.c_int8),
+                                            _p(height, C.c_uint8))
+        return td, last, height

@@ -1,0 +1,151 @@
+"""BottomUpHeuristicGPU on the MI355X against the oracle's restatement:
+every level of the history (vertex added last, height) byte-equal, the
+decomposition equal, partial runs and the queries with the reference's
+semantics."""
+import random
+from math import comb
+
+import numpy as np
+import pytest
+
+from gisbdw_amd import api
+from gisbdw_amd import graphs as G
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = api.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    return Oracle()
+
+
+def offsets(n):
+    off = [0]
+    for k in range(n + 1):
+        off.append(off[-1] + comb(n, k))
+    return off
+
+
+def graphs_small():
+    yield G.path(9)
+    yield G.cycle(11)
+    yield G.star(8)
+    yield G.complete(6)
+    yield G.edgeless(5)
+    yield G.grid_dropped(3, 5, 0.1, 3)
+    rng = random.Random(99)
+    for _ in range(14):
+        n = rng.randint(1, 16)
+        yield G.gnp(n, rng.choice([0.1, 0.2, 0.3, 0.5, 0.8]), rng.getrandbits(32))
+
+
+def test_history_bit_exact(ctx, oracle):
+    bu = api.BottomUpHeuristicGPU(ctx)
+    for g in graphs_small():
+        print("bu history", g.name, flush=True)
+        td, order, parent, root, last, height = oracle.bottom_up(g)
+        bu(g)
+        assert bu.iterations == g.n + 1
+        assert bu.treedepth == td, g.name
+        assert bu.parent == parent, g.name
+        off = offsets(g.n)
+        for k in range(g.n + 1):
+            gl, gh = bu.level(k)
+            assert np.array_equal(gl, last[off[k]: off[k + 1]]), (g.name, k)
+            assert np.array_equal(gh, height[off[k]: off[k + 1]]), (g.name, k)
+
+
+@pytest.mark.parametrize("n,p,seed", [(20, 0.2, 1), (22, 0.15, 2), (24, 0.12, 3)])
+def test_larger_graphs_tree_and_queries(ctx, oracle, n, p, seed):
+    g = G.gnp(n, p, seed)
+    print("bu", g.name, flush=True)
+    td, order, parent, root, last, height = oracle.bottom_up(g)
+    bu = api.BottomUpHeuristicGPU(ctx)
+    bu(g)
+    assert (bu.treedepth, bu.parent) == (td, parent)
+    assert bu.GetElimination(n, n, 0) == order
+    off = offsets(n)
+    rng = random.Random(seed)
+    for _ in range(20):
+        k = rng.randint(0, n)
+        code = rng.randrange(comb(n, k))
+        assert bu.GetTreedepth(n, k, code) == height[off[k] + code]
+        # GetElimination of the subset: the stored last vertex, removed, k times
+        S = 0
+        dec = oracle.decode(n, k, code)
+        for v in dec:
+            S |= 1 << v
+        want = []
+        for _ in range(k):
+            kk = bin(S).count("1")
+            v = int(last[off[kk] + oracle.encode([u for u in range(n) if S >> u & 1])])
+            want.append(v)
+            S &= ~(1 << v)
+        assert bu.GetElimination(n, k, code) == want
+
+
+def test_upper_bound_at_n28(ctx, oracle):
+    g = G.gnp(28, 0.15, 5)
+    print("bu n28", flush=True)
+    bu = api.BottomUpHeuristicGPU(ctx)
+    bu(g)
+    ex, _ = oracle.dp_fast(g)
+    assert bu.treedepth >= ex
+    assert oracle.certify(g, bu.parent, bu.treedepth)
+    assert bu.GetTreedepth(28, 28, 0) == bu.treedepth
+
+
+def test_partial_run_and_queries(ctx, oracle):
+    g = G.gnp(14, 0.3, 8)
+    bu = api.BottomUpHeuristicGPU(ctx)
+    bu(g, iterations=5)
+    assert bu.iterations == 5 and bu.treedepth == -1
+    _, _, _, _, last, height = oracle.bottom_up(g, iterations=5)
+    off = offsets(14)
+    for k in range(5):
+        gl, gh = bu.level(k)
+        assert np.array_equal(gl, last[off[k]: off[k + 1]])
+        assert np.array_equal(gh, height[off[k]: off[k + 1]])
+    assert bu.GetTreedepth(14, 7, 3) == 0  # beyond the levels run, like the reference
+    with pytest.raises(api.TDError) as e:
+        bu.GetElimination(14, 6, 0)
+    assert e.value.code == api.TD_E_STATE
+    r = bu.Get(g)  # full run again through Get
+    assert r.treedepth == oracle.bottom_up(g)[0]
+
+
+def test_memory_cap_reduces_levels(oracle):
+    # a context budget too small for every level: as many levels as fit,
+    # treedepth -1 (the reference's GetMaxIterations and UINT_MAX)
+    small = api.Context(0, mem_budget_bytes=1 << 20)
+    g = G.gnp(20, 0.2, 4)
+    bu = api.BottomUpHeuristicGPU(small)
+    bu(g)
+    assert 1 <= bu.iterations < 21 and bu.treedepth == -1
+    _, _, _, _, last, height = oracle.bottom_up(g, iterations=bu.iterations)
+    off = offsets(20)
+    k = bu.iterations - 1
+    gl, gh = bu.level(k)
+    assert np.array_equal(gh, height[off[k]: off[k + 1]])
+    small.close()
+
+
+def test_errors(ctx):
+    bu = api.BottomUpHeuristicGPU(api.Context(0))
+    with pytest.raises(api.TDError) as e:
+        bu.GetTreedepth(0, 0, 0)
+    assert e.value.code == api.TD_E_STATE
+    with pytest.raises(api.TDError):
+        bu(G.path(33))
+    with pytest.raises(api.TDError):
+        bu(G.path(8), iterations=0)
+    with pytest.raises(api.TDError):
+        bu(G.path(8), iterations=10)
