@@ -33,7 +33,7 @@ after td_dp_autotune_batch; and the oracle's CPU paths for the same
 configurations (faithful DynamicCPU, flat-table DP, BranchAndBound) on
 bounded samples.
 
-Other workloads: --workload dp20 | dpbatch.
+Other workloads: --workload dp20 | dpbatch | bottomup.
 """
 from __future__ import annotations
 
@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="dp30", choices=["dp30", "dp20", "dpbatch"])
+    ap.add_argument("--workload", default="dp30", choices=["dp30", "dp20", "dpbatch", "bottomup"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--batch-mode", type=int, default=0)
     ap.add_argument("--low-bits", type=int, default=None)
@@ -238,7 +238,23 @@ def gpu_configs_extra(ctx):
     c3_tuned = timed()
     for k, v in saved.items():
         ctx.set_param(k, v)
+    # the bottom-up heuristic (BottomUpHeuristicGPU) on the same G(20,0.3)
+    # and on a G(26,0.15): every level, then the walk and the forest
+    bu = api.BottomUpHeuristicGPU(ctx)
+    bu_out = {}
+    for name, gb in (("g20", g), ("g26", G.gnp(26, 0.15, 1))):
+        bu(gb)
+        reps = 5 if gb.n <= 20 else 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            bu(gb)
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        dp(gb)
+        bu_out[name] = {"graph": gb.name, "n": gb.n, "ms_end_to_end": ms,
+                        "subsets_per_s": (1 << gb.n) / (ms * 1e-3),
+                        "bottom_up_td": bu.treedepth, "exact_td": dp.treedepth}
     return {"dp_n20_single": {"treedepth": dp.treedepth, "ms_per_graph_end_to_end": c1_ms},
+            "bottom_up": bu_out,
             "dp_batch4096": {"graphs_per_s_before_tuning": len(batch) / c3_default,
                              "before_tuning": saved,
                              "graphs_per_s_autotuned": len(batch) / c3_tuned,
@@ -291,6 +307,12 @@ def cpu_extra(threads):
     out["dp_n20_single"] = {
         "faithful_dyncpu_1core": {"vertices": m, "s": dt, "subsets_per_s": (1 << m) / dt},
         "flat_dp_1core_ms": flat1 * 1e3, f"flat_dp_{threads}cores_ms": flatn * 1e3}
+    # the bottom-up heuristic's restatement (union-find arrays per subset,
+    # bottom_up_heuristic_gpu_algorithm.cu:57-95) on one core, same graph
+    t0 = time.perf_counter()
+    o.bottom_up(g, history=False)
+    dt = time.perf_counter() - t0
+    out["bottom_up_g20_1core"] = {"s": dt, "subsets_per_s": (1 << g.n) / dt}
 
     sample = G.workload_dp_batch(4096, 0)[:256]
     t0 = time.perf_counter()
@@ -310,6 +332,35 @@ def cpu_extra(threads):
                      "certified": cert, "certified_graphs_per_s": cert / dt,
                      "nodes_per_s": sum(r[4] for r in res) / dt}
     return out
+
+
+def cpu_baseline_bottom_up(g):
+    """The oracle's BottomUpHeuristicGPU restatement (union-find arrays per
+    subset, bottom_up_heuristic_gpu_algorithm.cu:57-95) on one core, over the
+    subgraph induced by the first m vertices of g (m picked for about 10 s)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib
+    from gisbdw_amd import graphs as G
+
+    o = oracle_lib.Oracle()
+
+    def induced(m):
+        mask = (1 << m) - 1
+        return G.Graph(m, [g.adj[v] & mask for v in range(m)], f"{g.name}[:{m}]")
+
+    m = 16
+    t0 = time.perf_counter()
+    o.bottom_up(induced(m), history=False)
+    dt = time.perf_counter() - t0
+    while m < g.n and dt * 2.3 < 10.0:
+        m += 1
+        dt *= 2.3
+    t0 = time.perf_counter()
+    o.bottom_up(induced(m), history=False)
+    dt = time.perf_counter() - t0
+    return {"value": (1 << m) / dt, "unit": "subsets/s", "cores": 1, "kind": "port",
+            "sample": f"oracle bottom-up restatement on the subgraph induced by vertices "
+                      f"0..{m - 1} of {g.name}: 2^{m} subsets in {dt:.2f} s"}
 
 
 def shape_key(ctx):
@@ -454,6 +505,41 @@ def main():
                 "compulsory_GBps": 2 * sub / (lvl_ms * 1e-3) / 1e9,
             },
         }
+    elif args.workload == "bottomup":
+        # BottomUpHeuristicGPU on one G(26,0.15): every level, then the walk
+        # and the forest.  Algorithmic bytes per subset S (|S| = k): the kept
+        # predecessor's 32-byte record and 34 bytes written, plus k height
+        # bytes and 2|E(S)| neighbour bytes read; over all subsets
+        # 66 * 2^n + (n + m) * 2^(n-1).
+        g = G.gnp(26, 0.15, 1)
+        graphs = [g]
+        m = sum(bin(r).count("1") for r in g.adj) // 2
+        alg_bytes = 66 * (1 << g.n) + (g.n + m) * (1 << (g.n - 1))
+        bu = api.BottomUpHeuristicGPU(ctx)
+        for _ in range(args.warmup):
+            bu(g)
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            bu(g)
+            print(f"[bench] step {i} td={bu.treedepth}", file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        achieved = alg_bytes * args.steps / wall / 1e9
+        out = {
+            "metric": "bottom-up heuristic subsets/sec (n=26)",
+            "value": (1 << g.n) * args.steps / wall,
+            "unit": "subsets/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": wall / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u8", "data": "synthetic",
+            "config": {"workload": f"bottom-up heuristic: {g.name}, {m} edges, all 27 levels",
+                       "graphs": 1, "n": g.n},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                         "kernel": "BottomUpLevelKernel", "alg_bytes_per_step": alg_bytes,
+                         "timing": "wall clock of the whole call"},
+            "extra": {"treedepth": bu.treedepth},
+        }
     else:
         graphs = G.workload_dp_batch(4096, 0)
         ns, adj = api.pack_graphs(graphs)  # inputs packed once, outside the timing
@@ -505,7 +591,10 @@ def main():
         out["extra"]["bnb"] = bnb_extra(ctx)
     if args.workload == "dp30" and not args.no_extra:
         out["extra"]["gpu_configs"] = gpu_configs_extra(ctx)
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "bottomup":
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline_bottom_up(graphs[0])
+    elif not args.no_cpu:
         g0 = graphs[0]
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline_dp(g0)

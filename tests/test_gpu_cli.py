@@ -43,3 +43,13 @@ def test_cli_heuristics(alg, kind, tmp_path, oracle):
         td, parent, _ = oracle.heuristic(g, kind)
         assert int(out["Treedepth"]) == td and out["Certified"] == "0", g.name
         assert sorted(edges) == sorted((v, p) for v, p in enumerate(parent) if p >= 0), g.name
+
+
+def test_cli_bottom_up(tmp_path, oracle):
+    """bottomUpHeur (Treedepth/main.cpp:203-204): the oracle's bottom-up depth
+    and forest, reported as uncertified."""
+    for g in (G.cycle(11), G.grid(3, 4), G.gnp_connected(14, 0.3, 2)):
+        out, edges = run("bottomUpHeur", g, tmp_path)
+        td, order, parent, root, _, _ = oracle.bottom_up(g, history=False)
+        assert int(out["Treedepth"]) == td and out["Certified"] == "0", g.name
+        assert sorted(edges) == sorted((v, p) for v, p in enumerate(parent) if p >= 0), g.name
