@@ -375,13 +375,16 @@ def dp_solve_batch_arrays(ctx: Context, ns: np.ndarray, adj: np.ndarray, mode: i
     return td, par, st
 
 
-HEURISTIC_KINDS = {"chain": 1, "highest_degree": 2, "spanning_tree": 3, "variance": 4}
+HEURISTIC_KINDS = {"chain": 1, "highest_degree": 2, "spanning_tree": 3, "variance": 4,
+                   "chain_bottom_up": 5, "chain_bnbgpu": 6}
 
 
 def heuristic_batch(ctx: Context, graphs: Sequence[Graph], kind="chain"):
     """td_heuristic_batch: the reference's upper-bound heuristics on the GPU.
     kind: "chain" (HighestDegree(SpanningTree(Variance))), "highest_degree",
-    "spanning_tree" or "variance".  Returns (depths, parent lists)."""
+    "spanning_tree", "variance", "chain_bottom_up" (HighestDegree(SpanningTree(
+    Variance(BottomUp)))) or "chain_bnbgpu" (HighestDegree(Variance(BottomUp))).
+    Returns (depths, parent lists)."""
     ns, adj = pack_graphs(graphs)
     td = np.zeros(len(ns), np.int32)
     par = np.zeros(int(ns.sum()), np.int8)

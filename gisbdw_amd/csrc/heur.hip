@@ -5,8 +5,11 @@
 //   VarianceHeuristic (variance_heuristic.hpp:24-78),
 //   SpanningTreeHeuristic (spanning_tree_heuristic.hpp:20-83) over the
 //     union-find spanning forest of common/utils/graph_gen.hpp:47-62,
-//   and the chain HighestDegree(SpanningTree(Variance(1.0, 0.2, 0.8))), whose
-//   decorators keep the inner result only when it is strictly shallower.
+//   and the chains HighestDegree(SpanningTree(Variance(1.0, 0.2, 0.8))),
+//   HighestDegree(SpanningTree(Variance(BottomUp))) and
+//   HighestDegree(Variance(BottomUp)) (Treedepth/main.cpp:172-186), whose
+//   decorators keep the inner result only when it is strictly shallower; the
+//   bottom-up link comes from td_bottom_up (bottomup.hip).
 // Each heuristic eliminates one vertex at a time from the first component
 // (lowest vertex first, the EliminationTree's component order) and the
 // result is the EliminationTree decomposition: the parent of v is the vertex
@@ -224,12 +227,15 @@ struct HeurTree {
 // HighestDegree, 3 = SpanningTree, 4 = Variance(1.0, 0.2, 0.8).
 __global__ __launch_bounds__(64) void HeuristicKernel(const int32_t* __restrict__ ns,
                                                       const uint64_t* __restrict__ adj64,
-                                                      int kind, int32_t* __restrict__ td_out,
+                                                      int kind,
+                                                      const int32_t* __restrict__ inner_td,
+                                                      const int8_t* __restrict__ inner_par,
+                                                      int32_t* __restrict__ td_out,
                                                       int8_t* __restrict__ parent_out) {
   __shared__ uint64_t s_adj[64], s_st[64];
   __shared__ int s_uf[64];
   __shared__ uint8_t s_sizes[64 * 64];
-  __shared__ int8_t s_par[3][64];
+  __shared__ int8_t s_par[4][64];
   const int gi = blockIdx.x, lane = threadIdx.x;
   const int n = ns[gi];
   const uint64_t row = lane < n ? adj64[static_cast<size_t>(gi) * 64 + lane] : 0ull;
@@ -245,9 +251,38 @@ __global__ __launch_bounds__(64) void HeuristicKernel(const int32_t* __restrict_
     td = SpanningTree(row, n, s_st, s_par[0]);
   } else if (kind == 4) {
     td = Variance(row, n, s_adj, s_sizes, A, D, E, s_par[0]);
+  } else if (kind == 6) {
+    // HighestDegree(Variance(inner)): decorators keep the inner result only
+    // when strictly shallower; a missing inner result (-1) never is
+    int tv = Variance(row, n, s_adj, s_sizes, A, D, E, s_par[0]);
+    int8_t* vt = s_par[0];
+    const int ti = inner_td[gi];
+    if (ti >= 0 && ti < tv) {
+      tv = ti;
+      s_par[3][lane] = lane < n ? inner_par[static_cast<size_t>(gi) * 64 + lane] : -1;
+      vt = s_par[3];
+    }
+    __syncthreads();
+    const int th = HighestDegree(row, n, s_par[2]);
+    td = th;
+    out = s_par[2];
+    if (tv < th) {
+      td = tv;
+      out = vt;
+    }
   } else {
-    // decorators keep the inner result only when strictly shallower
-    const int tv = Variance(row, n, s_adj, s_sizes, A, D, E, s_par[0]);
+    // kind 1: HighestDegree(SpanningTree(Variance)); kind 5: the same over an
+    // inner result.  Decorators keep the inner result only when strictly
+    // shallower.
+    int tv = Variance(row, n, s_adj, s_sizes, A, D, E, s_par[0]);
+    if (kind == 5) {
+      const int ti = inner_td[gi];
+      if (ti >= 0 && ti < tv) {
+        tv = ti;
+        s_par[0][lane] = lane < n ? inner_par[static_cast<size_t>(gi) * 64 + lane] : -1;
+      }
+      __syncthreads();
+    }
     SpanningForest(s_adj, n, s_st, s_uf);
     int ts = SpanningTree(row, n, s_st, s_par[1]);
     int8_t* st = s_par[1];
@@ -298,7 +333,7 @@ int td_heuristic_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
   return HGuard(ctx, [&] {
     TD_REQUIRE(ngraphs >= 1 && n && adj_concat && td_out, TD_E_INVALID,
                "empty batch or null pointer");
-    TD_REQUIRE(kind >= 1 && kind <= 4, TD_E_INVALID, "kind must be 1, 2, 3 or 4");
+    TD_REQUIRE(kind >= 1 && kind <= 6, TD_E_INVALID, "kind must be in [1, 6]");
     std::vector<size_t> off(static_cast<size_t>(ngraphs) + 1, 0);
     for (int i = 0; i < ngraphs; ++i) {
       TD_REQUIRE(n[i] >= 1 && n[i] <= kMaxHeurN, TD_E_INVALID, "graphs need n in [1, 64]");
@@ -317,8 +352,27 @@ int td_heuristic_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
     TD_HIP(hipMemcpyAsync(dn, n, 4ull * ngraphs, hipMemcpyHostToDevice, ctx->stream));
     TD_HIP(hipMemcpyAsync(dadj, rows.data(), rows.size() * 8, hipMemcpyHostToDevice,
                           ctx->stream));
+    // kinds 5 and 6: the bottom-up heuristic is the innermost link; it runs
+    // for graphs of at most 32 vertices (a larger graph's levels never fit,
+    // so the reference's link returns UINT_MAX there)
+    std::vector<int32_t> itd(static_cast<size_t>(ngraphs), -1);
+    std::vector<int8_t> ipar(kind >= 5 ? 64ull * ngraphs : 1, -1);
+    if (kind >= 5) {
+      for (int i = 0; i < ngraphs; ++i) {
+        if (n[i] > 32) continue;
+        int32_t t = -1;
+        const int rc = td_bottom_up(ctx, n[i], adj_concat + off[i], -1, &t,
+                                    ipar.data() + 64ull * i, nullptr);
+        if (rc != TD_OK) throw Error{rc, ctx->err};
+        itd[i] = t;
+      }
+    }
+    auto* ditd = static_cast<int32_t*>(ctx->Buffer("heur.inner_td", 4ull * ngraphs));
+    auto* dipar = static_cast<int8_t*>(ctx->Buffer("heur.inner_parent", ipar.size()));
+    TD_HIP(hipMemcpyAsync(ditd, itd.data(), 4ull * ngraphs, hipMemcpyHostToDevice, ctx->stream));
+    TD_HIP(hipMemcpyAsync(dipar, ipar.data(), ipar.size(), hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(HeuristicKernel, dim3(ngraphs), dim3(64), 0, ctx->stream, dn, dadj,
-                       kind, dtd, dpar);
+                       kind, ditd, dipar, dtd, dpar);
     TD_HIP(hipGetLastError());
     std::vector<int8_t> par(64ull * ngraphs);
     TD_HIP(hipMemcpyAsync(td_out, dtd, 4ull * ngraphs, hipMemcpyDeviceToHost, ctx->stream));

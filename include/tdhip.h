@@ -145,7 +145,13 @@ int td_dp_autotune_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
  *           Treedepth/src/branch_and_bound/branch_and_bound.hpp:38-42);
  *   kind 2: HighestDegreeHeuristic::Get (highest_degree_heuristic.hpp:19-45);
  *   kind 3: SpanningTreeHeuristic::Get (spanning_tree_heuristic.hpp:20-83);
- *   kind 4: VarianceHeuristic::Get (variance_heuristic.hpp:24-78).
+ *   kind 4: VarianceHeuristic::Get (variance_heuristic.hpp:24-78);
+ *   kind 5: HighestDegree(SpanningTree(Variance(BottomUpHeuristicGPU))), the
+ *           bnbCPU chain of Treedepth/main.cpp:172-176;
+ *   kind 6: HighestDegree(Variance(BottomUpHeuristicGPU)), the bnbGPU chain
+ *           of Treedepth/main.cpp:180-183.
+ * In kinds 5 and 6 the bottom-up link is td_bottom_up for graphs of at most
+ * 32 vertices and absent (the reference's UINT_MAX) above.
  * td_out[i] = depth of the tree, parent_concat (nullable) its parent array
  * (-1 = root).  Upper bounds, not proofs. */
 int td_heuristic_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,

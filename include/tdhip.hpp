@@ -173,7 +173,14 @@ class BnBGPU {
 // (branch_and_bound.hpp:38-42).
 class HeuristicGPU {
  public:
-  enum Kind { kChain = 1, kHighestDegree = 2, kSpanningTree = 3, kVariance = 4 };
+  enum Kind {
+    kChain = 1,
+    kHighestDegree = 2,
+    kSpanningTree = 3,
+    kVariance = 4,
+    kChainBottomUp = 5,  // HighestDegree(SpanningTree(Variance(BottomUp)))
+    kChainBnBGPU = 6     // HighestDegree(Variance(BottomUp))
+  };
   explicit HeuristicGPU(Kind kind = kChain,
                         std::shared_ptr<Context> ctx = std::make_shared<Context>())
       : kind_(kind), ctx_(std::move(ctx)) {}

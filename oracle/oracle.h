@@ -85,7 +85,10 @@ int32_t or_brute_td(int32_t n, const uint64_t* adj);
 /* lb_kind: 0 = EdgeLowerBound, 1 = BasicLowerBound.
  * heur_chain: 0 = none (initial best treedepth = n+1, the trivial path
  * bound), 1 = HighestDegree(SpanningTree(Variance(nullptr, 1.0,0.2,0.8))),
- * 2 = HighestDegree only, 3 = SpanningTree only, 4 = Variance only.
+ * 2 = HighestDegree only, 3 = SpanningTree only, 4 = Variance only,
+ * 5 = HighestDegree(SpanningTree(Variance(BottomUp))), 6 =
+ * HighestDegree(Variance(BottomUp)) (BottomUp = or_bottom_up for n <= 32,
+ * absent above).
  * node_budget: 0 = unlimited; otherwise stop after that many Algorithm()
  * calls and return -1 in *certified.  Returns treedepth; parent/root of the
  * best tree. Connected input only. */

@@ -4,6 +4,7 @@
 //
 // TEST INFRASTRUCTURE ONLY: used by tests/ and bench.py's cpu_baseline leg.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <memory>
@@ -145,6 +146,25 @@ static Res SpanningTreeH(int n, const uint64_t* adj) {
   return FromTree(res);
 }
 
+// BottomUpHeuristicGPU::Get (bottom_up_heuristic.hpp:17-31): every level for
+// n <= 32; above, its levels never fit and it returns UINT_MAX.
+static Res BottomUpRes(int n, const uint64_t* adj) {
+  Res r;
+  r.parent.assign(n, -1);
+  r.root = 0;
+  r.treedepth = INT_MAX;
+  if (n > 32) return r;
+  std::vector<int32_t> order(n), parent(n);
+  int32_t root = 0;
+  const int32_t td = or_bottom_up(n, adj, n + 1, nullptr, nullptr, order.data(), parent.data(),
+                                  &root);
+  if (td < 0) return r;
+  for (int v = 0; v < n; ++v) r.parent[v] = parent[v];
+  r.treedepth = td;
+  r.root = root;
+  return r;
+}
+
 static Res Heuristic(int n, const uint64_t* adj, int chain) {
   // Decorators return the inner result only when strictly better.
   switch (chain) {
@@ -154,12 +174,27 @@ static Res Heuristic(int n, const uint64_t* adj, int chain) {
       return SpanningTreeH(n, adj);
     case 4:
       return Variance(n, adj, 1.0f, 0.2f, 0.8f);
-    case 1: {
+    case 1:
+    case 5: {
+      // 5: the bnbCPU chain of main.cpp:172-176, BottomUpHeuristicGPU innermost
       Res var = Variance(n, adj, 1.0f, 0.2f, 0.8f);
+      if (chain == 5) {
+        Res bu = BottomUpRes(n, adj);
+        if (bu.treedepth < var.treedepth) var = bu;
+      }
       Res st = SpanningTreeH(n, adj);
       if (var.treedepth < st.treedepth) st = var;
       Res hd = HighestDegree(n, adj);
       if (st.treedepth < hd.treedepth) hd = st;
+      return hd;
+    }
+    case 6: {
+      // the bnbGPU chain of main.cpp:180-183
+      Res var = Variance(n, adj, 1.0f, 0.2f, 0.8f);
+      Res bu = BottomUpRes(n, adj);
+      if (bu.treedepth < var.treedepth) var = bu;
+      Res hd = HighestDegree(n, adj);
+      if (var.treedepth < hd.treedepth) hd = var;
       return hd;
     }
     default: {

@@ -46,3 +46,26 @@ def test_heuristic_errors(ctx):
         api.heuristic_batch(ctx, [G.path(5)], 7)
     with pytest.raises(api.TDError):
         api.heuristic_batch(ctx, [G.Graph(3, [0b010, 0, 0])], 2)
+
+
+@pytest.mark.parametrize("kind,code", [("chain_bottom_up", 5), ("chain_bnbgpu", 6)])
+def test_chains_with_bottom_up_equal_oracle(ctx, oracle, kind, code):
+    """The reference CLI's chains with BottomUpHeuristicGPU innermost
+    (Treedepth/main.cpp:172-186): the bottom-up link runs for n <= 32 and is
+    absent above; on about one graph in six of this family it is strictly
+    better than Variance and decides the result."""
+    rng = random.Random(77)
+    gs = [G.gnp(rng.randint(6, 16), rng.uniform(0.1, 0.5), rng.getrandbits(32))
+          for _ in range(60)]
+    gs += [G.path(15), G.cycle(13), G.grid(3, 5), G.edgeless(4), G.gnp(40, 0.1, 3),
+           G.random_cubic(40, 5)]
+    tds, pars = api.heuristic_batch(ctx, gs, kind)
+    taken = 0
+    for g, t, p in zip(gs, tds, pars):
+        want_td, want_par, _ = oracle.heuristic(g, code)
+        print(kind, g.name, g.n, t, want_td, flush=True)
+        assert t == want_td, (kind, g.name)
+        assert p == want_par, (kind, g.name)
+        assert oracle.certify(g, p, t), (kind, g.name)
+        taken += t < oracle.heuristic(g, 1 if code == 5 else 4)[0]
+    assert taken > 0
