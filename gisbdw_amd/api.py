@@ -320,6 +320,8 @@ class BottomUpHeuristicGPU:
         return self
 
     def Get(self, g: Graph) -> Result:
+        if g.n > 32:  # its levels never fit: the reference returns UINT_MAX
+            return Result([], 2 ** 32 - 1, 0, [], [])
         self(g)
         if self.treedepth < 0:
             return Result([], 2 ** 32 - 1, 0, [], [])

@@ -157,8 +157,10 @@ class BnBGPU {
     return td;
   }
 
-  std::pair<Result, bool> Solve(const Graph& g) {
-    const int td = (*this)(g, 0);
+  // The search from an optional upper-bound seed (heur_td, 0 = none): the
+  // decomposition and whether the search completed.
+  std::pair<Result, bool> Solve(const Graph& g, std::size_t heur_td = 0) {
+    const int td = (*this)(g, heur_td);
     return {Result::FromParent(parent_, static_cast<unsigned>(td)), certified_};
   }
 
@@ -249,6 +251,11 @@ class BottomUpHeuristicGPU {
   explicit BottomUpHeuristicGPU(std::shared_ptr<Context> ctx = std::make_shared<Context>())
       : algo_(std::move(ctx)) {}
   Result Get(const Graph& g) {
+    if (g.n > 32) {  // its levels never fit: the reference returns UINT_MAX
+      Result r;
+      r.treedepth = std::numeric_limits<unsigned>::max();
+      return r;
+    }
     algo_(g);
     if (algo_.Treedepth() < 0) {
       Result r;

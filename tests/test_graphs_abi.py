@@ -108,5 +108,6 @@ def test_cpp_cli_built_and_fails_loudly_without_gpu(tmp_path):
         pytest.skip("a GPU may be present")
     f = tmp_path / "p3.gviz"
     f.write_text("graph {\n0;\n1;\n2;\n0 -- 1;\n1 -- 2;\n}\n")
-    r = subprocess.run([str(cli), "-a", "dynGPU", str(f)], capture_output=True, text=True)
+    r = subprocess.run([str(cli), "-a", "dynGPU", "-o", str(tmp_path), str(f)],
+                       capture_output=True, text=True)
     assert r.returncode == 1 and "error 3" in r.stderr
