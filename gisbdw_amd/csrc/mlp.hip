@@ -64,43 +64,47 @@ __host__ __device__ inline float FromBf16(uint16_t h) {
 
 __device__ __forceinline__ void Sync() { __syncthreads(); }
 
-// acc[tm][tn][i] holds D[16 tm + 4 (lane/16) + i][16 tn + lane%16].
-struct Acc {
-  v4f t[4][4];
+// A graph's 64x64 products are split over the workgroup's 4 waves: wave w
+// computes rows 16w..16w+15 (tile-row w) of every product.
+constexpr int kWaves = 4;
+constexpr int kBlock = 64 * kWaves;
+
+__device__ __forceinline__ int Lane() { return threadIdx.x & 63; }
+__device__ __forceinline__ int WaveId() { return threadIdx.x >> 6; }
+
+// acc.t[tn][i] holds D[16 tm + 4 (lane/16) + i][16 tn + lane%16] of tile-row tm.
+struct AccRow {
+  v4f t[4];
 };
 
-__device__ __forceinline__ void Zero(Acc& a) {
+__device__ __forceinline__ void Zero(AccRow& a) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) a.t[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int j = 0; j < 4; ++j) a.t[j] = v4f{0.f, 0.f, 0.f, 0.f};
 }
 
-// acc += A (64xK=64) * B (64x64); A(m,k) = A[m*am + k*ak], B(k,n) =
-// B[k*bk + n*bn], bf16 bit patterns.  Lane l supplies row/col l%16 and the
-// k-group l/16 (8 consecutive k) of each 16x16x32 step.  kAc / kBc: ak == 1
-// / bk == 1, so a lane's 8 k-values are 16 contiguous, 16-byte aligned
-// bytes and come in one load.
+// acc += A (rows 16tm..16tm+15 of a 64x64 A) * B (64x64); A(m,k) =
+// A[m*am + k*ak], B(k,n) = B[k*bk + n*bn], bf16 bit patterns.  Lane l
+// supplies row/col l%16 and the k-group l/16 (8 consecutive k) of each
+// 16x16x32 step.  kAc / kBc: ak == 1 / bk == 1, so a lane's 8 k-values are
+// 16 contiguous, 16-byte aligned bytes and come in one load.
 template <bool kAc, bool kBc>
-__device__ __forceinline__ void MatMul(Acc& acc, const uint16_t* __restrict__ A,
-                                       int am, int ak,
-                                       const uint16_t* __restrict__ B, int bk,
-                                       int bn) {
-  const int lane = threadIdx.x & 63;
+__device__ __forceinline__ void MatMulRow(AccRow& acc, int tm, const uint16_t* __restrict__ A,
+                                          int am, int ak, const uint16_t* __restrict__ B,
+                                          int bk, int bn) {
+  const int lane = Lane();
   const int r = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int ks = 0; ks < 2; ++ks) {
-    v8s a[4], b[4];
+    v8s a, b[4];
     const int k0 = 32 * ks + 8 * g;
+    if constexpr (kAc) {
+      a = __builtin_bit_cast(v8s, *reinterpret_cast<const uint4*>(A + (16 * tm + r) * am + k0));
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a[j] = static_cast<short>(A[(16 * tm + r) * am + (k0 + j) * ak]);
+    }
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      if constexpr (kAc) {
-        a[t] = __builtin_bit_cast(v8s, *reinterpret_cast<const uint4*>(A + (16 * t + r) * am + k0));
-      } else {
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-          a[t][j] = static_cast<short>(A[(16 * t + r) * am + (k0 + j) * ak]);
-      }
       if constexpr (kBc) {
         b[t] = __builtin_bit_cast(v8s, *reinterpret_cast<const uint4*>(B + (16 * t + r) * bn + k0));
       } else {
@@ -110,53 +114,43 @@ __device__ __forceinline__ void MatMul(Acc& acc, const uint16_t* __restrict__ A,
       }
     }
 #pragma unroll
-    for (int tm = 0; tm < 4; ++tm)
-#pragma unroll
-      for (int tn = 0; tn < 4; ++tn)
-        acc.t[tm][tn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-            __builtin_bit_cast(v8bf, a[tm]), __builtin_bit_cast(v8bf, b[tn]),
-            acc.t[tm][tn], 0, 0, 0);
+    for (int tn = 0; tn < 4; ++tn)
+      acc.t[tn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+          __builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b[tn]), acc.t[tn], 0, 0, 0);
   }
 }
 
 // element (row, col) owned by this lane for tile (tm, tn), register i
-__device__ __forceinline__ int AccRow(int tm, int i) {
-  return 16 * tm + 4 * ((threadIdx.x & 63) >> 4) + i;
-}
+__device__ __forceinline__ int AccRowIdx(int tm, int i) { return 16 * tm + 4 * (Lane() >> 4) + i; }
 __device__ __forceinline__ int AccCol(int tn) { return 16 * tn + (threadIdx.x & 15); }
 
-// Y = relu(acc + bias[col]) -> bf16 at Y[row*64 + col]
-__device__ __forceinline__ void StoreReluBf16(const Acc& acc, const float* bias,
+// Y = relu(acc + bias[col]) -> bf16 at Y[row*kLd + col], tile-row tm
+__device__ __forceinline__ void StoreReluBf16(const AccRow& acc, int tm, const float* bias,
                                               uint16_t* Y) {
 #pragma unroll
-  for (int tm = 0; tm < 4; ++tm)
+  for (int tn = 0; tn < 4; ++tn)
 #pragma unroll
-    for (int tn = 0; tn < 4; ++tn)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int row = AccRow(tm, i), col = AccCol(tn);
-        const float v = acc.t[tm][tn][i] + bias[col];
-        Y[row * kLd + col] = ToBf16(v > 0.f ? v : 0.f);
-      }
+    for (int i = 0; i < 4; ++i) {
+      const int row = AccRowIdx(tm, i), col = AccCol(tn);
+      const float v = acc.t[tn][i] + bias[col];
+      Y[row * kLd + col] = ToBf16(v > 0.f ? v : 0.f);
+    }
 }
 
-__device__ __forceinline__ void StoreF32(const Acc& acc, float* Y, int ym, int yn) {
+__device__ __forceinline__ void StoreF32(const AccRow& acc, int tm, float* Y, int ym, int yn) {
 #pragma unroll
-  for (int tm = 0; tm < 4; ++tm)
+  for (int tn = 0; tn < 4; ++tn)
 #pragma unroll
-    for (int tn = 0; tn < 4; ++tn)
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        Y[AccRow(tm, i) * ym + AccCol(tn) * yn] = acc.t[tm][tn][i];
+    for (int i = 0; i < 4; ++i) Y[AccRowIdx(tm, i) * ym + AccCol(tn) * yn] = acc.t[tn][i];
 }
 
-// m[v] = mean over neighbours u of h[u] (fp32 sum of bf16 values), rounded
-__device__ __forceinline__ void Aggregate(const uint16_t* h, uint16_t* m,
-                                          uint64_t myadj) {
-  const int v = threadIdx.x & 63;
+// m[v][c] = mean over neighbours u of h[u][c] (fp32 sum of bf16 values),
+// rounded; lane v, this wave's 16 columns
+__device__ __forceinline__ void Aggregate(const uint16_t* h, uint16_t* m, uint64_t myadj) {
+  const int v = Lane(), c0 = 16 * WaveId();
   const int deg = __builtin_popcountll(myadj);
   const float inv = deg ? 1.0f / static_cast<float>(deg) : 0.f;
-  for (int c = 0; c < kH; ++c) {
+  for (int c = c0; c < c0 + 16; ++c) {
     float s = 0.f;
     for (uint64_t a = myadj; a; a &= a - 1) s += FromBf16(h[__builtin_ctzll(a) * kLd + c]);
     m[v * kLd + c] = ToBf16(s * inv);
@@ -181,11 +175,11 @@ __host__ __device__ inline int TransposedIndex(int p) {
 
 __device__ __forceinline__ float Wb(const Net& N, int p) { return FromBf16(N.wb[p]); }
 
-// Forward for one graph (this wave); activations to S (LDS); returns s_v in
-// lane v.
+// Forward for one graph (this workgroup); activations to S (LDS); returns
+// s_v in lane v of every wave.  s_red: 4 x 64 floats of scratch.
 __device__ float Forward(const Net& N, const float* __restrict__ x, uint64_t myadj,
-                         uint16_t* S) {
-  const int v = threadIdx.x & 63;
+                         uint16_t* S, float* s_red) {
+  const int v = Lane(), w = WaveId(), c0 = 16 * w;
   uint16_t* h0 = S + A_H0 * kMatLd;
   uint16_t* m1 = S + A_M1 * kMatLd;
   uint16_t* h1 = S + A_H1 * kMatLd;
@@ -194,7 +188,7 @@ __device__ float Forward(const Net& N, const float* __restrict__ x, uint64_t mya
   uint16_t* z1 = S + A_Z1 * kMatLd;
   uint16_t* z2 = S + A_Z2 * kMatLd;
   const float xv[4] = {x[v * 4 + 0], x[v * 4 + 1], x[v * 4 + 2], x[v * 4 + 3]};
-  for (int o = 0; o < kH; ++o) {
+  for (int o = c0; o < c0 + 16; ++o) {
     float a = N.w[P_B0 + o];
 #pragma unroll
     for (int i = 0; i < 4; ++i) a += Wb(N, P_W0 + o * 4 + i) * xv[i];
@@ -203,29 +197,35 @@ __device__ float Forward(const Net& N, const float* __restrict__ x, uint64_t mya
   Sync();
   Aggregate(h0, m1, myadj);
   Sync();
-  Acc acc;
+  AccRow acc;
   Zero(acc);
-  MatMul<true, true>(acc, h0, kLd, 1, N.wb + P_WS1, 1, kH);
-  MatMul<true, true>(acc, m1, kLd, 1, N.wb + P_WN1, 1, kH);
-  StoreReluBf16(acc, N.w + P_B1, h1);
+  MatMulRow<true, true>(acc, w, h0, kLd, 1, N.wb + P_WS1, 1, kH);
+  MatMulRow<true, true>(acc, w, m1, kLd, 1, N.wb + P_WN1, 1, kH);
+  StoreReluBf16(acc, w, N.w + P_B1, h1);
   Sync();
   Aggregate(h1, m2, myadj);
   Sync();
   Zero(acc);
-  MatMul<true, true>(acc, h1, kLd, 1, N.wb + P_WS2, 1, kH);
-  MatMul<true, true>(acc, m2, kLd, 1, N.wb + P_WN2, 1, kH);
-  StoreReluBf16(acc, N.w + P_B2, h2);
+  MatMulRow<true, true>(acc, w, h1, kLd, 1, N.wb + P_WS2, 1, kH);
+  MatMulRow<true, true>(acc, w, m2, kLd, 1, N.wb + P_WN2, 1, kH);
+  StoreReluBf16(acc, w, N.w + P_B2, h2);
   Sync();
   Zero(acc);
-  MatMul<true, true>(acc, h2, kLd, 1, N.wb + P_W3, 1, kH);
-  StoreReluBf16(acc, N.w + P_B3, z1);
+  MatMulRow<true, true>(acc, w, h2, kLd, 1, N.wb + P_W3, 1, kH);
+  StoreReluBf16(acc, w, N.w + P_B3, z1);
   Sync();
   Zero(acc);
-  MatMul<true, true>(acc, z1, kLd, 1, N.wb + P_W4, 1, kH);
-  StoreReluBf16(acc, N.w + P_B4, z2);
+  MatMulRow<true, true>(acc, w, z1, kLd, 1, N.wb + P_W4, 1, kH);
+  StoreReluBf16(acc, w, N.w + P_B4, z2);
   Sync();
-  float s = N.w[P_B5];
-  for (int j = 0; j < kH; ++j) s += FromBf16(z2[v * kLd + j]) * Wb(N, P_W5 + j);
+  // s_v = b5 + w5 . z2[v]: this wave's 16 terms, then the 4 partial sums in
+  // wave order
+  float part = 0.f;
+  for (int j = c0; j < c0 + 16; ++j) part += FromBf16(z2[v * kLd + j]) * Wb(N, P_W5 + j);
+  s_red[w * 64 + v] = part;
+  Sync();
+  const float s = N.w[P_B5] + (((s_red[v] + s_red[64 + v]) + s_red[128 + v]) + s_red[192 + v]);
+  Sync();
   return s;
 }
 
@@ -298,41 +298,46 @@ __global__ __launch_bounds__(64) void MLPFeatureKernel(const int32_t* __restrict
   Features(myadj, n, feat + static_cast<size_t>(gi) * 256);
 }
 
-__global__ __launch_bounds__(64) void MLPScoreKernel(Net N, const int32_t* __restrict__ ns,
-                                                     const uint64_t* __restrict__ adj,
-                                                     const float* __restrict__ feat,
-                                                     float* __restrict__ scores) {
+__global__ __launch_bounds__(kBlock) void MLPScoreKernel(Net N, const int32_t* __restrict__ ns,
+                                                         const uint64_t* __restrict__ adj,
+                                                         const float* __restrict__ feat,
+                                                         float* __restrict__ scores) {
   __shared__ uint16_t s_act[kScoreMats * kMatLd];
-  const int gi = blockIdx.x, v = threadIdx.x;
+  __shared__ float s_red[kWaves * 64];
+  const int gi = blockIdx.x, v = Lane();
   const int n = ns[gi];
   const uint64_t myadj = v < n ? adj[static_cast<size_t>(gi) * 64 + v] : 0ull;
-  const float s = Forward(N, feat + static_cast<size_t>(gi) * 256, myadj, s_act);
-  scores[static_cast<size_t>(gi) * 64 + v] = v < n ? s : 0.f;
+  const float s = Forward(N, feat + static_cast<size_t>(gi) * 256, myadj, s_act, s_red);
+  if (WaveId() == 0) scores[static_cast<size_t>(gi) * 64 + v] = v < n ? s : 0.f;
 }
 
-// Forward + backward of one graph per wave.  Gradient of
-// scale * loss_i goes to part[i][*] (one row per graph, reduced later).
-__global__ __launch_bounds__(64) void MLPTrainKernel(
+// Forward + backward of one graph per workgroup (4 waves; wave w owns rows
+// 16w..16w+15 of every product and columns 16w..16w+15 of every element-wise
+// step).  Gradient of scale * loss_i goes to part[i][*] (one row per graph,
+// reduced later).  Sums split over the waves are combined in wave order.
+__global__ __launch_bounds__(kBlock) void MLPTrainKernel(
     Net N, const int32_t* __restrict__ ns, const uint64_t* __restrict__ adj,
     const uint64_t* __restrict__ targets, const float* __restrict__ feat,
     const int32_t* __restrict__ index, float* __restrict__ part, float* __restrict__ loss,
     float scale) {
   const int bi = blockIdx.x;
   const int gi = index ? index[bi] : bi;
-  const int v = threadIdx.x;
+  const int v = Lane(), w = WaveId(), c0 = 16 * w;
   const int n = ns[gi];
   const bool in = v < n;
   const uint64_t myadj = in ? adj[static_cast<size_t>(gi) * 64 + v] : 0ull;
   __shared__ uint16_t s_act[kNBf16 * kMatLd];
   __shared__ float s_f[2 * kMatLdF];
   __shared__ float s_degf[64];
-  s_degf[v] = static_cast<float>(__builtin_popcountll(myadj));  // read after Forward's syncs
+  __shared__ float s_red[kWaves * 64 * 5];
+  if (w == 0) s_degf[v] = static_cast<float>(__builtin_popcountll(myadj));  // read after Forward's syncs
   uint16_t* S = s_act;
   float* SF = s_f;
   float* G = part + static_cast<size_t>(bi) * kNParams;
   const float* x = feat + static_cast<size_t>(gi) * 256;
-  const float s = Forward(N, x, myadj, S);
+  const float s = Forward(N, x, myadj, S, s_red);
   // ---- loss: softmax over the graph's vertices vs uniform optimal roots
+  // (every wave computes it; wave 0 writes it)
   const uint64_t tgt = targets[gi];
   const int npos = __builtin_popcountll(tgt);
   const float mx = WaveMaxF(in ? s : -INFINITY);
@@ -342,7 +347,7 @@ __global__ __launch_bounds__(64) void MLPTrainKernel(
   const float t = ((tgt >> v) & 1ull) ? 1.0f / static_cast<float>(npos) : 0.f;
   const float li = (t > 0.f) ? -t * (s - mx - logf(z)) : 0.f;
   const float lsum = WaveSumF(li);
-  if (v == 0) loss[bi] = lsum;
+  if (w == 0 && v == 0) loss[bi] = lsum;
   const float ds = in ? (p - t) * scale : 0.f;  // dL/ds_v
   uint16_t* h0 = S + A_H0 * kMatLd;
   uint16_t* m1 = S + A_M1 * kMatLd;
@@ -355,87 +360,87 @@ __global__ __launch_bounds__(64) void MLPTrainKernel(
   float* F0 = SF;                   // fp32 gradient w.r.t. an activation
   float* F1 = SF + kMatLdF;
   // ---- head: s = w5 . z2 + b5
-  // dw5[j] = sum_r ds_r z2[r][j] (lane j); db5 = sum_r ds_r
+  // dw5[j] = sum_r ds_r z2[r][j] (lane j, rows of this wave); db5 = sum_r ds_r
   {
     float dw = 0.f;
-    for (int r = 0; r < kH; ++r) dw += __shfl(ds, r) * FromBf16(z2[r * kLd + v]);
-    G[P_W5 + v] = dw;
-    const float db = WaveSumF(ds);
-    if (v == 0) G[P_B5] = db;
-    // da4[r][j] = ds_r w5[j] (z2 > 0)  (lane r)
-    for (int j = 0; j < kH; ++j) {
+    for (int r = c0; r < c0 + 16; ++r) dw += __shfl(ds, r) * FromBf16(z2[r * kLd + v]);
+    s_red[w * 64 + v] = dw;
+    // da4[r][j] = ds_r w5[j] (z2 > 0)  (lane r, this wave's columns j)
+    for (int j = c0; j < c0 + 16; ++j) {
       const float zz = FromBf16(z2[v * kLd + j]);
       gA[v * kLd + j] = ToBf16(zz > 0.f ? ds * Wb(N, P_W5 + j) : 0.f);
     }
   }
   Sync();
+  if (w == 0) {
+    G[P_W5 + v] = ((s_red[v] + s_red[64 + v]) + s_red[128 + v]) + s_red[192 + v];
+    const float db = WaveSumF(ds);
+    if (v == 0) G[P_B5] = db;
+  }
   // dense layer backward: given gA = d pre-activation (bf16, [r][o]),
   // input X ([r][i]) and weight W ([o][i]): dW = gA^T X, db = colsum gA,
-  // dX (fp32, [r][i]) = gA W into Fout.
+  // dX (fp32, [r][i]) = gA W into Fout.  Ends with a barrier.
   auto dense_back = [&](const uint16_t* Xin, int pW, int pB, float* Fout) {
-    Acc acc;
-    Zero(acc);
-    MatMul<false, false>(acc, gA, 1, kLd, Xin, kLd, 1);  // (o, r) x (r, i)
-    StoreF32(acc, G + pW, kH, 1);
     float db = 0.f;
-    for (int r = 0; r < kH; ++r) db += FromBf16(gA[r * kLd + v]);
-    G[pB + v] = db;
+    for (int r = c0; r < c0 + 16; ++r) db += FromBf16(gA[r * kLd + v]);
+    Sync();  // s_red is free again
+    s_red[w * 64 + v] = db;
+    AccRow acc;
+    Zero(acc);
+    MatMulRow<false, false>(acc, w, gA, 1, kLd, Xin, kLd, 1);  // (o, r) x (r, i)
+    StoreF32(acc, w, G + pW, kH, 1);
     if (Fout) {
       Zero(acc);
-      MatMul<true, true>(acc, gA, kLd, 1, N.wt + pW, 1, kH);  // (r, o) x (o, i)
-      StoreF32(acc, Fout, kLdF, 1);
+      MatMulRow<true, true>(acc, w, gA, kLd, 1, N.wt + pW, 1, kH);  // (r, o) x (o, i)
+      StoreF32(acc, w, Fout, kLdF, 1);
     }
+    Sync();
+    if (w == 0) G[pB + v] = ((s_red[v] + s_red[64 + v]) + s_red[128 + v]) + s_red[192 + v];
+  };
+  // d pre-activation of a relu layer: gA = (y > 0) * F0, this wave's columns
+  auto relu_grad = [&](const uint16_t* y) {
+    for (int j = c0; j < c0 + 16; ++j) {
+      const float yy = FromBf16(y[v * kLd + j]);
+      gA[v * kLd + j] = ToBf16(yy > 0.f ? F0[v * kLdF + j] : 0.f);
+    }
+    Sync();
   };
   // layer 4: z2 = relu(W4 z1 + b4)
   dense_back(z1, P_W4, P_B4, F0);
-  Sync();
-  for (int j = 0; j < kH; ++j) {
-    const float y = FromBf16(z1[v * kLd + j]);
-    gA[v * kLd + j] = ToBf16(y > 0.f ? F0[v * kLdF + j] : 0.f);
-  }
-  Sync();
+  relu_grad(z1);
   // layer 3: z1 = relu(W3 h2 + b3)
   dense_back(h2, P_W3, P_B3, F0);
-  Sync();
-  for (int j = 0; j < kH; ++j) {
-    const float y = FromBf16(h2[v * kLd + j]);
-    gA[v * kLd + j] = ToBf16(y > 0.f ? F0[v * kLdF + j] : 0.f);
-  }
-  Sync();
+  relu_grad(h2);
   // message-passing layer: h_out = relu(Ws h_in + Wn m + b), m = mean_N h_in
   auto mp_back = [&](const uint16_t* hin, const uint16_t* mag, int pWs, int pWn, int pB) {
     dense_back(hin, pWs, pB, F0);  // dWs, db, F0 = gA Ws
-    Acc acc;
+    AccRow acc;
     Zero(acc);
-    MatMul<false, false>(acc, gA, 1, kLd, mag, kLd, 1);
-    StoreF32(acc, G + pWn, kH, 1);
+    MatMulRow<false, false>(acc, w, gA, 1, kLd, mag, kLd, 1);
+    StoreF32(acc, w, G + pWn, kH, 1);
     Zero(acc);
-    MatMul<true, true>(acc, gA, kLd, 1, N.wt + pWn, 1, kH);
-    StoreF32(acc, F1, kLdF, 1);  // dL/dm
+    MatMulRow<true, true>(acc, w, gA, kLd, 1, N.wt + pWn, 1, kH);
+    StoreF32(acc, w, F1, kLdF, 1);  // dL/dm
     Sync();
     // d h_in[u] += sum_{v in N(u)} dm[v] / deg_v   (lane u; degrees from LDS)
-    for (int c = 0; c < kH; ++c) {
+    for (int c = c0; c < c0 + 16; ++c) {
       float a = F0[v * kLdF + c];
       for (uint64_t nb = myadj; nb; nb &= nb - 1) {
-        const int w = __builtin_ctzll(nb);
-        a += F1[w * kLdF + c] / s_degf[w];
+        const int u = __builtin_ctzll(nb);
+        a += F1[u * kLdF + c] / s_degf[u];
       }
       F0[v * kLdF + c] = a;
     }
     Sync();
   };
   mp_back(h1, m2, P_WS2, P_WN2, P_B2);
-  for (int j = 0; j < kH; ++j) {
-    const float y = FromBf16(h1[v * kLd + j]);
-    gA[v * kLd + j] = ToBf16(y > 0.f ? F0[v * kLdF + j] : 0.f);
-  }
-  Sync();
+  relu_grad(h1);
   mp_back(h0, m1, P_WS1, P_WN1, P_B1);
-  // layer 0: h0 = relu(W0 x + b0), computed in fp32 from the fp32 d h0
+  // layer 0: h0 = relu(W0 x + b0), computed in fp32 from the fp32 d h0.
+  // Lane o accumulates dW0[o][0..3] and db0[o] over this wave's rows r.
   {
-    // lane o accumulates dW0[o][0..3] and db0[o] over rows r
     float dw[4] = {0.f, 0.f, 0.f, 0.f}, db = 0.f;
-    for (int r = 0; r < kH; ++r) {
+    for (int r = c0; r < c0 + 16; ++r) {
       const float y = FromBf16(h0[r * kLd + v]);
       const float da = y > 0.f ? F0[r * kLdF + v] : 0.f;
       db += da;
@@ -443,8 +448,18 @@ __global__ __launch_bounds__(64) void MLPTrainKernel(
       for (int i = 0; i < 4; ++i) dw[i] += da * x[r * 4 + i];
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) G[P_W0 + v * 4 + i] = dw[i];
-    G[P_B0 + v] = db;
+    for (int i = 0; i < 4; ++i) s_red[(w * 5 + i) * 64 + v] = dw[i];
+    s_red[(w * 5 + 4) * 64 + v] = db;
+    Sync();
+    if (w == 0) {
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        const float a = ((s_red[i * 64 + v] + s_red[(5 + i) * 64 + v]) + s_red[(10 + i) * 64 + v]) +
+                        s_red[(15 + i) * 64 + v];
+        if (i < 4) G[P_W0 + v * 4 + i] = a;
+        else G[P_B0 + v] = a;
+      }
+    }
   }
 }
 
@@ -701,7 +716,7 @@ int td_mlp_scores(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint64_
     DevNet d = NetBuffers(ctx);
     Uploaded u = UploadGraphs(ctx, "mlp.in", ngraphs, ns, adj64);
     auto* sc = static_cast<float*>(ctx->Buffer("mlp.scores", 256ull * ngraphs));
-    hipLaunchKernelGGL(MLPScoreKernel, dim3(ngraphs), dim3(64), 0, ctx->stream,
+    hipLaunchKernelGGL(MLPScoreKernel, dim3(ngraphs), dim3(kBlock), 0, ctx->stream,
                        Net{d.w, d.wb, d.wt}, u.ns, u.adj, u.feat, sc);
     TD_HIP(hipGetLastError());
     TD_HIP(hipMemcpyAsync(scores_out, sc, 256ull * ngraphs, hipMemcpyDeviceToHost,
@@ -729,7 +744,7 @@ int td_mlp_loss_grad(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint
         ctx->Buffer("mlp.part", 4ull * kNParams * static_cast<size_t>(ngraphs)));
     auto* loss = static_cast<float*>(ctx->Buffer("mlp.loss", 4ull * ngraphs));
     auto* grad = static_cast<float*>(ctx->Buffer("mlp.grad", 4ull * kNParams));
-    hipLaunchKernelGGL(MLPTrainKernel, dim3(ngraphs), dim3(64), 0, ctx->stream,
+    hipLaunchKernelGGL(MLPTrainKernel, dim3(ngraphs), dim3(kBlock), 0, ctx->stream,
                        Net{d.w, d.wb, d.wt}, u.ns, u.adj, tg, u.feat, nullptr, part, loss,
                        1.0f / static_cast<float>(ngraphs));
     hipLaunchKernelGGL(MLPReduceKernel, dim3((kNParams + kReduceParams - 1) / kReduceParams),
@@ -783,7 +798,7 @@ int td_mlp_train(td_ctx* ctx, int32_t ngraphs, const int32_t* ns, const uint64_t
                             ctx->stream));
       int nb = 0;
       for (int s0 = 0; s0 + batch <= ngraphs; s0 += batch, ++nb) {
-        hipLaunchKernelGGL(MLPTrainKernel, dim3(batch), dim3(64), 0, ctx->stream,
+        hipLaunchKernelGGL(MLPTrainKernel, dim3(batch), dim3(kBlock), 0, ctx->stream,
                            Net{d.w, d.wb, d.wt}, u.ns, u.adj, tg, u.feat, idx + s0, part, loss + s0,
                            1.0f / static_cast<float>(batch));
         hipLaunchKernelGGL(MLPReduceKernel,
