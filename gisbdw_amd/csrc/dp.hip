@@ -574,13 +574,17 @@ __global__ void ResetCounter(uint32_t* c) {
 }
 
 // td of subset S (raw byte >> 1) of graph g.
-// Binomials and, for a class layout, the first tile of each class per high
-// level: s_cbase[16 h + p] = ClassBase(h, p, nh - cb).  Ends with a barrier.
+// Binomials, the level offsets (read per lane, so LDS spares a memory round
+// trip before each table read) and, for a class layout, the first tile of
+// each class per high level: s_cbase[16 h + p] = ClassBase(h, p, nh - cb),
+// all in LDS.  s_cbase[kCbaseWords..] holds lvl[].  Ends with a barrier.
 constexpr int kCbaseWords = 33 * 16;
+constexpr int kLookupWords = kCbaseWords + 34;
 __device__ __forceinline__ void LoadLookupTables(const DPGraph* __restrict__ g,
                                                  const uint32_t* __restrict__ g_cb,
                                                  uint32_t* s_cb, uint32_t* s_cbase) {
   for (int i = threadIdx.x; i < 32 * kCbStride; i += blockDim.x) s_cb[i] = g_cb[i];
+  for (int i = threadIdx.x; i < 34; i += blockDim.x) s_cbase[kCbaseWords + i] = g->lvl[i];
   __syncthreads();
   const int cb = static_cast<int>(g->cb), nh = static_cast<int>(g->nh);
   if (cb > 0)
@@ -607,7 +611,7 @@ __device__ __forceinline__ uint32_t Lookup(const DPGraph* __restrict__ g,
     Hr &= Hr - 1;
   }
   const size_t addr =
-      g->table + ((static_cast<size_t>(g->lvl[h]) + r) << lb) + L;
+      g->table + ((static_cast<size_t>(s_cbase[kCbaseWords + h]) + r) << lb) + L;
   return static_cast<uint32_t>(arena[addr]) >> 1;
 }
 
@@ -628,7 +632,7 @@ __global__ __launch_bounds__(64) void DPWalkKernel(
     const int32_t* __restrict__ nsteps, int32_t* __restrict__ orders,
     int order_stride) {
   __shared__ uint32_t s_cb[32 * kCbStride];
-  __shared__ uint32_t s_cbase[kCbaseWords];
+  __shared__ uint32_t s_cbase[kLookupWords];
   const DPGraph* g = gs + blockIdx.x;
   LoadLookupTables(g, g_cb, s_cb, s_cbase);
   const int lane = threadIdx.x;
@@ -679,7 +683,7 @@ __global__ __launch_bounds__(64) void DPTreeKernel(
     const uint8_t* __restrict__ arena, const uint32_t* __restrict__ start,
     int8_t* __restrict__ parent_out, int32_t* __restrict__ td_out) {
   __shared__ uint32_t s_cb[32 * kCbStride];
-  __shared__ uint32_t s_cbase[kCbaseWords];
+  __shared__ uint32_t s_cbase[kLookupWords];
   const DPGraph* g = gs + blockIdx.x;
   LoadLookupTables(g, g_cb, s_cb, s_cbase);
   const int lane = threadIdx.x;
@@ -730,7 +734,7 @@ __global__ __launch_bounds__(256) void DPExportLevelKernel(
     const uint8_t* __restrict__ arena, int k, uint32_t count,
     uint8_t* __restrict__ out) {
   __shared__ uint32_t s_cb[32 * kCbStride];
-  __shared__ uint32_t s_cbase[kCbaseWords];
+  __shared__ uint32_t s_cbase[kLookupWords];
   LoadLookupTables(g, g_cb, s_cb, s_cbase);
   const int n = static_cast<int>(g->n);
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < count;
