@@ -151,7 +151,8 @@ int td_dp_autotune_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
  *   kind 6: HighestDegree(Variance(BottomUpHeuristicGPU)), the bnbGPU chain
  *           of Treedepth/main.cpp:180-183.
  * In kinds 5 and 6 the bottom-up link is td_bottom_up for graphs of at most
- * 32 vertices and absent (the reference's UINT_MAX) above.
+ * 32 vertices and absent (the reference's UINT_MAX) above; the context's
+ * td_bottom_up queries then refer to the last such graph of the batch.
  * td_out[i] = depth of the tree, parent_concat (nullable) its parent array
  * (-1 = root).  Upper bounds, not proofs. */
 int td_heuristic_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
