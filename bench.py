@@ -507,10 +507,12 @@ def main():
         }
     elif args.workload == "bottomup":
         # BottomUpHeuristicGPU on one G(26,0.15): every level, then the walk
-        # and the forest.  Algorithmic bytes per subset S (|S| = k): the kept
-        # predecessor's 32-byte record and 34 bytes written, plus k height
-        # bytes and 2|E(S)| neighbour bytes read; over all subsets
-        # 66 * 2^n + (n + m) * 2^(n-1).
+        # and the forest.  Algorithmic bytes per subset S (|S| = k), one byte
+        # per height: the kept forest's 32 slots read and 34 bytes written
+        # (its 32 slots, the vertex added last, the height), plus the k
+        # heights of S \ v and 2|E(S)| neighbour slots read; over all subsets
+        # 66 * 2^n + (n + m) * 2^(n-1).  The kernel packs slots into 4 bits
+        # where the heights allow, so it moves fewer bytes than this.
         g = G.gnp(26, 0.15, 1)
         graphs = [g]
         m = sum(bin(r).count("1") for r in g.adj) // 2

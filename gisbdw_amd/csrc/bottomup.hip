@@ -12,13 +12,19 @@
 //
 // Only the component heights matter for later levels (Union reads the heights
 // of the two roots; Find only the root), and the components of the kept forest
-// are those of G[S].  So a subset's record here is the height of each vertex's
-// component: 32 bytes (vertex w at byte w, 0 outside S) instead of the
-// reference's n + 2 union-find bytes.  Adding v: height 1 + the largest
-// height among v's neighbours in S (1 if none); the forest height is the
-// larger of that and the height of S \ v.  The history keeps, per subset, the
-// vertex added last and the height (history_ in the reference), 2 bytes per
-// subset over all levels; the records live for two levels only.
+// are those of G[S].  So a subset's record here holds 32 slots, one per
+// vertex: slot w is the height of w's component for w in S, and the height of
+// the whole forest for w outside S.  Adding v to S \ v then needs one record:
+// slot v gives the height of S \ v, v's neighbours' slots the heights that v
+// joins (the new component's height is 1 + their largest, 1 if none), and the
+// forest height is the larger of the two.  A level's heights are at most k and
+// at most 1 + the largest height of the level below, so a level whose heights
+// must stay below 16 stores 4-bit slots (16-byte records), any other level
+// bytes (32-byte records), instead of the reference's n + 2 union-find bytes;
+// each launch finds the width from the largest heights the launches before it
+// recorded.  The history keeps, per subset, the vertex added last and the
+// height (history_ in the reference), 2 bytes per subset over all levels; the
+// records live for two levels only.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -29,21 +35,32 @@ namespace tdhip {
 namespace {
 
 constexpr int kBuStride = 33;  // s_cb[x * 33 + j] = C(x, j)
-constexpr int kRecBytes = 32;  // per-subset record: component height per vertex
+constexpr int kRecBytes = 32;  // largest per-subset record (byte slots)
+constexpr int kNibLevels = 15; // levels k <= 15 always fit 4-bit slots
 constexpr int kBuBlock = 256;
+// The binomial buffer continues with O_k = sum_{j<k} C(16, j) (k = 0..16)
+// and, from word kBuTailWord, the colex k-subsets of {0..15} as 16-bit masks
+// (entry O_k + r is the subset of rank r).
+constexpr int kBuTailBits = 16;
+constexpr int kBuOffs = 32 * kBuStride;
+constexpr int kBuCbWords = kBuOffs + kBuTailBits + 1;
+constexpr int kBuTailWord = (kBuCbWords + 31) / 32 * 32;
+constexpr int kBuPascalWords = kBuTailWord + (1 << kBuTailBits) / 2;
 
-// Colex unrank of r among the k-subsets of {0..n-1} (set_encoder.hpp:61-73),
-// greedy from the top; needs r < C(n, k).
-__device__ __forceinline__ uint32_t BuUnrank(uint32_t r, int k, int n, const uint32_t* s_cb) {
+// Colex unrank of r among the k-subsets of {0..n-1} (set_encoder.hpp:61-73):
+// greedy from the top down to vertex 16, then one table read for the rest
+// (the remainder is then below C(16, k)); needs r < C(n, k).
+__device__ __forceinline__ uint32_t BuUnrank(uint32_t r, int k, int n, const uint32_t* s_cb,
+                                             const uint16_t* __restrict__ tail) {
   uint32_t S = 0;
-  for (int x = n - 1; x >= 0; --x) {
+  for (int x = n - 1; x >= kBuTailBits; --x) {
     const uint32_t c = s_cb[x * kBuStride + k];
     const uint32_t take = r >= c ? 1u : 0u;
     r = take ? r - c : r;
     S = 2u * S + take;
     k -= static_cast<int>(take);
   }
-  return S;
+  return (S << kBuTailBits) | tail[s_cb[kBuOffs + k] + r];
 }
 
 // Connected component of v in G[S] (breadth-first on bitmask rows).
@@ -58,69 +75,238 @@ __device__ __forceinline__ uint32_t BuComponent(const uint32_t* s_adj, uint32_t 
   return comp;
 }
 
-// One level k (1 <= k <= n): every k-subset of rank r < count.
-__global__ __launch_bounds__(kBuBlock) void BottomUpLevelKernel(
-    const uint32_t* __restrict__ g_adj, const uint32_t* __restrict__ g_cb, int n, int k,
-    uint32_t count, const uint8_t* __restrict__ prev_rec,
-    const uint8_t* __restrict__ prev_height, uint8_t* __restrict__ cur_rec,
-    int8_t* __restrict__ cur_last, uint8_t* __restrict__ cur_height) {
-  __shared__ uint32_t s_cb[32 * kBuStride];
-  __shared__ uint32_t s_adj[32];
-  for (int i = threadIdx.x; i < 32 * kBuStride; i += blockDim.x) s_cb[i] = g_cb[i];
-  if (threadIdx.x < 32) s_adj[threadIdx.x] = threadIdx.x < static_cast<unsigned>(n) ? g_adj[threadIdx.x] : 0u;
-  __syncthreads();
+// A record in registers: a, b (4-bit slots, slot w at bits 4w of the
+// 128-bit value b:a) or a..d (byte slots, slot w at bits 8w of d:c:b:a).
+// Named fields, not an array, so that slot selects stay register selects.
+struct BuRec {
+  uint64_t a, b, c, d;
+};
+
+template <bool kNib>
+__device__ __forceinline__ BuRec BuLoad(const uint8_t* p) {
+  BuRec q;
+  const uint4 x = reinterpret_cast<const uint4*>(p)[0];
+  q.a = x.x | static_cast<uint64_t>(x.y) << 32;
+  q.b = x.z | static_cast<uint64_t>(x.w) << 32;
+  q.c = q.d = 0;
+  if (!kNib) {
+    const uint4 y = reinterpret_cast<const uint4*>(p)[1];
+    q.c = y.x | static_cast<uint64_t>(y.y) << 32;
+    q.d = y.z | static_cast<uint64_t>(y.w) << 32;
+  }
+  return q;
+}
+
+// Slot w; the words come as values (a reference would let the compiler turn
+// the selects into a dynamically indexed load from private memory).
+template <bool kNib>
+__device__ __forceinline__ int BuSlot(uint64_t a, uint64_t b, uint64_t c, uint64_t d, int w) {
+  if (kNib) {
+    const uint64_t x = (w & 16) ? b : a;
+    return static_cast<int>(x >> (4 * (w & 15))) & 15;
+  }
+  const uint64_t lo = (w & 8) ? b : a;
+  const uint64_t hi = (w & 8) ? d : c;
+  return static_cast<int>(((w & 16) ? hi : lo) >> (8 * (w & 7))) & 255;
+}
+
+// 32-bit word d (compile-time after unrolling) of a record
+__device__ __forceinline__ uint32_t BuWord(uint64_t a, uint64_t b, uint64_t c, uint64_t d,
+                                           int i) {
+  const uint64_t x = i < 2 ? a : i < 4 ? b : i < 6 ? c : d;
+  return static_cast<uint32_t>(x >> (32 * (i & 1)));
+}
+
+// bit i of the low 8 bits -> 0xF in nibble i (fills the LDS table)
+__device__ __forceinline__ uint32_t BuSpreadNib(uint32_t x) {
+  x &= 0xFFu;
+  x = (x | (x << 12)) & 0x000F000Fu;
+  x = (x | (x << 6)) & 0x03030303u;
+  x = (x | (x << 3)) & 0x11111111u;
+  return x * 15u;
+}
+
+// bit i of the low 4 bits -> 0xFF in byte i
+__device__ __forceinline__ uint32_t BuSpreadByte(uint32_t x) {
+  x &= 0xFu;
+  x = (x | (x << 14)) & 0x00030003u;
+  x = (x | (x << 7)) & 0x01010101u;
+  return x * 255u;
+}
+
+// 8 nibbles -> 8 bytes (nibble i to byte i)
+__device__ __forceinline__ uint64_t BuNibToBytes(uint32_t v) {
+  uint64_t x = v;
+  x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+  return x;
+}
+
+// 8 bytes, each below 16 -> 8 nibbles (byte i to nibble i)
+__device__ __forceinline__ uint32_t BuBytesToNib(uint64_t x) {
+  x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+  return static_cast<uint32_t>(x);
+}
+
+// Levels whose heights all stay below 16 keep 4-bit slots: level k's
+// heights are at most 1 + the largest height of level k - 1 (and at most k).
+__device__ __forceinline__ bool BuNibLevel(int k, const uint32_t* maxh) {
+  return k <= kNibLevels || maxh[k - 1] <= 14u;
+}
+
+// Every k-subset of rank r < count at one level; returns the largest height
+// this lane wrote.
+template <bool kNibIn, bool kNibOut>
+__device__ __forceinline__ uint32_t BuLevel(const uint32_t* s_cb, const uint32_t* s_adj,
+                                            const uint32_t* s_spread,
+                                            const uint16_t* __restrict__ tail, int n, int k,
+                                            uint32_t count, const uint8_t* __restrict__ prev_rec,
+                                            uint8_t* __restrict__ cur_rec,
+                                            int8_t* __restrict__ cur_last,
+                                            uint8_t* __restrict__ cur_height) {
+  constexpr int kIn = kNibIn ? 16 : 32, kOut = kNibOut ? 16 : 32;
+  uint32_t lane_max = 0;
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < count;
        r += gridDim.x * blockDim.x) {
-    const uint32_t S = BuUnrank(r, k, n, s_cb);
-    // rank(S \ y_i) = sum_{j<i} C(y_j, j+1) + sum_{j>i} C(y_j, j)
-    // (set_encoder.hpp:48-59), with y_0 < y_1 < ... the elements of S
-    uint32_t Q = 0;
-    int j = 0;
-    for (uint32_t m = S; m; m &= m - 1, ++j) Q += s_cb[__builtin_ctz(m) * kBuStride + j];
-    uint32_t P = 0, Qpre = 0, brank = 0;
+    const uint32_t S = BuUnrank(r, k, n, s_cb, tail);
+    // Elements y_0 < ... < y_{k-1} of S from the top: rank(S \ y_i) =
+    // sum_{j<i} C(y_j, j+1) + sum_{j>i} C(y_j, j) (set_encoder.hpp:48-59),
+    // where the first sum is r - sum_{j>=i} C(y_j, j+1).  Walking down with
+    // "<=" keeps the smallest vertex of least height, as the ascending
+    // strict "<" of the reference does.  The records of kGroup removals are
+    // loaded together; every lane has k elements, so the groups (and the
+    // `ok` flags) are the same across the wave.
+    constexpr int kGroup = kNibIn ? 4 : 2;
+    uint32_t sufa = 0, sufb = 0, brank = 0;
     int best = k + 1, bv = 0, bnew = 0;
-    j = 0;
-    for (uint32_t m = S; m; m &= m - 1, ++j) {
-      const int v = __builtin_ctz(m);
-      const uint32_t c1 = s_cb[v * kBuStride + j + 1];
-      Qpre += s_cb[v * kBuStride + j];
-      const uint32_t r2 = P + (Q - Qpre);
-      P += c1;
-      const uint8_t* rec = prev_rec + static_cast<size_t>(r2) * kRecBytes;
-      int top = 0;
-      for (uint32_t a = s_adj[v] & S; a; a &= a - 1)
-        top = max(top, static_cast<int>(rec[__builtin_ctz(a)]));
-      const int newh = top + 1;
-      // the history's height of the empty set is 0 where the reference's
-      // union-find state holds 1; newh >= 1 makes the two agree here
-      const int h = max(static_cast<int>(prev_height[r2]), newh);
-      if (h < best) {
-        best = h;
-        bv = v;
-        bnew = newh;
-        brank = r2;
+    int i = k - 1;
+    for (uint32_t m = S; m;) {
+      int vs[kGroup];
+      uint32_t r2s[kGroup];
+      bool ok[kGroup];
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        ok[u] = m != 0;
+        const int v = ok[u] ? 31 - __builtin_clz(m) : 0;
+        vs[u] = v;
+        r2s[u] = 0;  // record 0 of the previous level: a safe address
+        if (ok[u]) {
+          m ^= 1u << v;
+          sufb += s_cb[v * kBuStride + i + 1];
+          r2s[u] = r - sufb + sufa;
+          sufa += s_cb[v * kBuStride + i];
+          --i;
+        }
+      }
+      BuRec qs[kGroup];
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u)
+        qs[u] = BuLoad<kNibIn>(prev_rec + static_cast<size_t>(r2s[u]) * kIn);
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        if (!ok[u]) break;
+        const int v = vs[u];
+        const BuRec& q = qs[u];
+        int top = 0;
+        for (uint32_t a = s_adj[v] & S; a; a &= a - 1)
+          top = max(top, BuSlot<kNibIn>(q.a, q.b, q.c, q.d, __builtin_ctz(a)));
+        const int newh = top + 1;
+        // slot v of S \ v holds its forest height (0 for the empty set,
+        // where the reference's union-find state holds 1; newh >= 1 makes
+        // the two agree here)
+        const int h = max(BuSlot<kNibIn>(q.a, q.b, q.c, q.d, v), newh);
+        if (h <= best) {
+          best = h;
+          bv = v;
+          bnew = newh;
+          brank = r2s[u];
+        }
       }
     }
     const uint32_t comp = BuComponent(s_adj, S, bv);
-    const uint4* src = reinterpret_cast<const uint4*>(prev_rec + static_cast<size_t>(brank) * kRecBytes);
+    BuRec q = BuLoad<kNibIn>(prev_rec + static_cast<size_t>(brank) * kIn);
+    if (kNibIn && !kNibOut) {  // widen 4-bit slots to bytes
+      const uint64_t lo = q.a, hi = q.b;
+      q.a = BuNibToBytes(static_cast<uint32_t>(lo));
+      q.b = BuNibToBytes(static_cast<uint32_t>(lo >> 32));
+      q.c = BuNibToBytes(static_cast<uint32_t>(hi));
+      q.d = BuNibToBytes(static_cast<uint32_t>(hi >> 32));
+    } else if (!kNibIn && kNibOut) {  // narrow bytes (all below 15 here)
+      q.a = BuBytesToNib(q.a) | static_cast<uint64_t>(BuBytesToNib(q.b)) << 32;
+      q.b = BuBytesToNib(q.c) | static_cast<uint64_t>(BuBytesToNib(q.d)) << 32;
+    }
+    // kept slots: S outside v's component; v's component: bnew; outside S:
+    // the forest height
+    const uint32_t out_of_s = ~S;
     uint32_t w[8];
-    const uint4 a0 = src[0], a1 = src[1];
-    w[0] = a0.x; w[1] = a0.y; w[2] = a0.z; w[3] = a0.w;
-    w[4] = a1.x; w[5] = a1.y; w[6] = a1.z; w[7] = a1.w;
+    if (kNibOut) {
+      const uint32_t rep_new = static_cast<uint32_t>(bnew) * 0x11111111u;
+      const uint32_t rep_best = static_cast<uint32_t>(best) * 0x11111111u;
 #pragma unroll
-    for (int d = 0; d < 8; ++d) {
+      for (int d = 0; d < 4; ++d) {
+        const uint32_t in = BuWord(q.a, q.b, q.c, q.d, d);
+        const uint32_t cm = s_spread[(comp >> (8 * d)) & 0xFFu];
+        const uint32_t nm = s_spread[(out_of_s >> (8 * d)) & 0xFFu];
+        w[d] = (in & ~(cm | nm)) | (rep_new & cm) | (rep_best & nm);
+      }
+    } else {
+      const uint32_t rep_new = static_cast<uint32_t>(bnew) * 0x01010101u;
+      const uint32_t rep_best = static_cast<uint32_t>(best) * 0x01010101u;
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        if ((comp >> (4 * d + b)) & 1u)
-          w[d] = (w[d] & ~(0xFFu << (8 * b))) | (static_cast<uint32_t>(bnew) << (8 * b));
+      for (int d = 0; d < 8; ++d) {
+        const uint32_t in = BuWord(q.a, q.b, q.c, q.d, d);
+        const uint32_t cm = BuSpreadByte(comp >> (4 * d)), nm = BuSpreadByte(out_of_s >> (4 * d));
+        w[d] = (in & ~(cm | nm)) | (rep_new & cm) | (rep_best & nm);
       }
     }
-    uint4* dst = reinterpret_cast<uint4*>(cur_rec + static_cast<size_t>(r) * kRecBytes);
+    uint4* dst = reinterpret_cast<uint4*>(cur_rec + static_cast<size_t>(r) * kOut);
     dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    if (!kNibOut) dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
     cur_last[r] = static_cast<int8_t>(bv);
     cur_height[r] = static_cast<uint8_t>(best);
+    lane_max = max(lane_max, static_cast<uint32_t>(best));
   }
+  return lane_max;
+}
+
+// One level k (1 <= k <= n).  maxh[j] is the largest height of level j
+// (maxh[0] = 0, maxh[k] cleared before the launch); it picks the slot width
+// of level k - 1's records and of this level's, and gets level k's maximum.
+__global__ __launch_bounds__(kBuBlock) void BottomUpLevelKernel(
+    const uint32_t* __restrict__ g_adj, const uint32_t* __restrict__ g_cb, int n, int k,
+    uint32_t count, const uint8_t* __restrict__ prev_rec, uint8_t* __restrict__ cur_rec,
+    int8_t* __restrict__ cur_last, uint8_t* __restrict__ cur_height,
+    uint32_t* __restrict__ maxh) {
+  __shared__ uint32_t s_cb[kBuCbWords];
+  __shared__ uint32_t s_adj[32];
+  __shared__ uint32_t s_spread[256];
+  __shared__ uint32_t s_max;
+  for (int i = threadIdx.x; i < kBuCbWords; i += blockDim.x) s_cb[i] = g_cb[i];
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) s_spread[i] = BuSpreadNib(i);
+  if (threadIdx.x < 32) s_adj[threadIdx.x] = threadIdx.x < static_cast<unsigned>(n) ? g_adj[threadIdx.x] : 0u;
+  if (threadIdx.x == 0) s_max = 0;
+  __syncthreads();
+  const uint16_t* tail = reinterpret_cast<const uint16_t*>(g_cb + kBuTailWord);
+  const bool nib_in = BuNibLevel(k - 1, maxh), nib_out = BuNibLevel(k, maxh);
+  uint32_t lane_max;
+  if (nib_in && nib_out)
+    lane_max = BuLevel<true, true>(s_cb, s_adj, s_spread, tail, n, k, count, prev_rec, cur_rec,
+                                   cur_last, cur_height);
+  else if (nib_in)
+    lane_max = BuLevel<true, false>(s_cb, s_adj, s_spread, tail, n, k, count, prev_rec,
+                                    cur_rec, cur_last, cur_height);
+  else if (nib_out)
+    lane_max = BuLevel<false, true>(s_cb, s_adj, s_spread, tail, n, k, count, prev_rec,
+                                    cur_rec, cur_last, cur_height);
+  else
+    lane_max = BuLevel<false, false>(s_cb, s_adj, s_spread, tail, n, k, count, prev_rec,
+                                     cur_rec, cur_last, cur_height);
+  atomicMax(&s_max, lane_max);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(&maxh[k], s_max);
 }
 
 // GetElimination (bottom_up_heuristic_gpu_algorithm.hpp:92-108): from set S,
@@ -164,9 +350,17 @@ int BGuard(td_ctx* ctx, F&& f) {
 const uint32_t* BuPascal(td_ctx* ctx) {
   auto it = ctx->bufs.find("bu.pascal");
   if (it != ctx->bufs.end() && it->second.p) return static_cast<const uint32_t*>(it->second.p);
-  std::vector<uint32_t> cb(32 * kBuStride, 0);
+  std::vector<uint32_t> cb(kBuPascalWords, 0);
   for (int x = 0; x < 32; ++x)
     for (int j = 0; j < kBuStride; ++j) cb[x * kBuStride + j] = static_cast<uint32_t>(Binom(x, j));
+  uint16_t* tail = reinterpret_cast<uint16_t*>(cb.data() + kBuTailWord);
+  uint32_t off = 0;
+  for (int k = 0; k <= kBuTailBits; ++k) {
+    cb[kBuOffs + k] = off;
+    // the k-subsets of {0..15} in colex order = increasing integers
+    for (uint32_t mask = 0; mask < (1u << kBuTailBits); ++mask)
+      if (__builtin_popcount(mask) == k) tail[off++] = static_cast<uint16_t>(mask);
+  }
   void* d = ctx->Buffer("bu.pascal", cb.size() * 4);
   TD_HIP(hipMemcpy(d, cb.data(), cb.size() * 4, hipMemcpyHostToDevice));
   return static_cast<const uint32_t*>(d);
@@ -182,13 +376,13 @@ uint64_t BuBytes(int n, int K) {
     hist += Binom(n, k);
   }
   return 2 * maxc * kRecBytes + 2 * ((hist + 255) / 256 * 256) + 8 * 40 + 4 * 32 +
-         4 * 32 * kBuStride + 4 * 32;
+         4 * kBuPascalWords + 4 * 32 + 4 * 40;
 }
 
 uint64_t BuHeld(td_ctx* ctx) {
   uint64_t b = 0;
   for (const char* k : {"bu.rec0", "bu.rec1", "bu.last", "bu.height", "bu.off", "bu.adj",
-                        "bu.pascal", "bu.walk"}) {
+                        "bu.pascal", "bu.walk", "bu.maxh"}) {
     auto it = ctx->bufs.find(k);
     if (it != ctx->bufs.end()) b += it->second.bytes;
   }
@@ -259,23 +453,26 @@ int td_bottom_up(td_ctx* ctx, int32_t n, const uint64_t* adj, int32_t iterations
     uint8_t* height = static_cast<uint8_t*>(ctx->Buffer("bu.height", (hist + 255) / 256 * 256));
     uint64_t* doff = static_cast<uint64_t*>(ctx->Buffer("bu.off", 8 * 40));
     uint32_t* dadj = static_cast<uint32_t*>(ctx->Buffer("bu.adj", 4 * 32));
+    uint32_t* dmaxh = static_cast<uint32_t*>(ctx->Buffer("bu.maxh", 4 * 40));
     const uint32_t* cb = BuPascal(ctx);
     uint32_t hadj[32] = {};
     for (int v = 0; v < n; ++v) hadj[v] = static_cast<uint32_t>(adj[v]);
     TD_HIP(hipMemcpyAsync(dadj, hadj, sizeof(hadj), hipMemcpyHostToDevice, ctx->stream));
     TD_HIP(hipMemcpyAsync(doff, off.data(), 8 * off.size(), hipMemcpyHostToDevice, ctx->stream));
     // level 0: the empty set, history (-1, 0) (:232-234), record all zero
+    // (every slot is outside the set and holds its height, 0)
     TD_HIP(hipMemsetAsync(rec[0], 0, kRecBytes, ctx->stream));
     TD_HIP(hipMemsetAsync(last, 0xFF, 1, ctx->stream));
     TD_HIP(hipMemsetAsync(height, 0, 1, ctx->stream));
+    TD_HIP(hipMemsetAsync(dmaxh, 0, 4 * 40, ctx->stream));  // largest height per level
     for (int k = 1; k < K; ++k) {
       const uint32_t count = static_cast<uint32_t>(Binom(n, k));
       const int grid = static_cast<int>(std::min<uint64_t>(
           (count + kBuBlock - 1) / kBuBlock,
           static_cast<uint64_t>(ctx->prop.multiProcessorCount) * 32));
       hipLaunchKernelGGL(BottomUpLevelKernel, dim3(grid), dim3(kBuBlock), 0, ctx->stream, dadj,
-                         cb, n, k, count, rec[(k - 1) & 1], height + off[k - 1], rec[k & 1],
-                         last + off[k], height + off[k]);
+                         cb, n, k, count, rec[(k - 1) & 1], rec[k & 1], last + off[k],
+                         height + off[k], dmaxh);
       TD_HIP(hipGetLastError());
     }
     TD_HIP(hipStreamSynchronize(ctx->stream));

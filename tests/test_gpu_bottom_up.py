@@ -63,6 +63,25 @@ def test_history_bit_exact(ctx, oracle):
             assert np.array_equal(gh, height[off[k]: off[k + 1]]), (g.name, k)
 
 
+@pytest.mark.parametrize("make", [lambda: G.complete(18), lambda: G.gnp(19, 0.85, 7),
+                                  lambda: G.gnp(17, 0.7, 3)])
+def test_history_bit_exact_tall_forests(ctx, oracle, make):
+    # heights above 15: the records switch from 4-bit slots to bytes
+    g = make()
+    print("bu tall", g.name, flush=True)
+    td, order, parent, root, last, height = oracle.bottom_up(g)
+    bu = api.BottomUpHeuristicGPU(ctx)
+    bu(g)
+    assert (bu.treedepth, bu.parent) == (td, parent), g.name
+    off = offsets(g.n)
+    for k in range(g.n + 1):
+        gl, gh = bu.level(k)
+        assert np.array_equal(gl, last[off[k]: off[k + 1]]), (g.name, k)
+        assert np.array_equal(gh, height[off[k]: off[k + 1]]), (g.name, k)
+    if g.n == 18:  # K_n: every subset's forest is a path of its size
+        assert int(height.max()) == 18
+
+
 @pytest.mark.parametrize("n,p,seed", [(20, 0.2, 1), (22, 0.15, 2), (24, 0.12, 3)])
 def test_larger_graphs_tree_and_queries(ctx, oracle, n, p, seed):
     g = G.gnp(n, p, seed)
