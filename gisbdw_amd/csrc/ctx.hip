@@ -71,6 +71,7 @@ void* td_ctx::Pinned(size_t bytes) {
   if (pinned && pinned_bytes >= bytes) return pinned;
   if (pinned) {
     TD_HIP(hipStreamSynchronize(stream));
+    if (side) TD_HIP(hipStreamSynchronize(side));
     TD_HIP(hipHostFree(pinned));
     pinned = nullptr;
     pinned_bytes = 0;
@@ -85,6 +86,7 @@ void* td_ctx::Buffer(const std::string& name, size_t bytes) {
   if (b.bytes >= bytes && b.p) return b.p;
   if (b.p) {
     TD_HIP(hipStreamSynchronize(stream));
+    if (side) TD_HIP(hipStreamSynchronize(side));
     TD_HIP(hipFree(b.p));
     allocated -= b.bytes;
     b.p = nullptr;
@@ -192,12 +194,16 @@ void td_ctx_destroy(td_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->side) (void)hipStreamSynchronize(c->side);
   for (auto& kv : c->dp_graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto& kv : c->bufs)
     if (kv.second.p) (void)hipFree(kv.second.p);
   if (c->pinned) (void)hipHostFree(c->pinned);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->pev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -132,6 +132,11 @@ struct td_ctx {
   void* Pinned(size_t bytes);
 
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // second stream and events of the two-slot pipeline that overlaps one
+  // large graph's decomposition with the next graph's levels (created on
+  // first use)
+  hipStream_t side = nullptr;
+  hipEvent_t pev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 
   // branch-and-bound memo tables: entries are tagged per graph serial
   void* bnb_memo_ptr = nullptr;

@@ -1002,15 +1002,16 @@ void LaunchLevelsLB(td_ctx* ctx, const DPGraph& G, const DPGraph* dG,
 }
 
 // Level sequence of one graph, replayed from a captured hipGraph when the
-// same (n, launch shape, buffers) was seen before.
+// same (n, launch shape, buffers) was seen before.  `slot` tells apart the
+// table buffers of the two-slot pipeline, so each keeps its own graph.
 void RunLevels(td_ctx* ctx, const DPGraph& G, const DPGraph* dG,
-               const uint32_t* cb, uint8_t* arena, uint64_t* launches) {
+               const uint32_t* cb, uint8_t* arena, uint64_t* launches, int slot = 0) {
   if (!ctx->dp.use_graph) {
     LaunchLevelsLB(ctx, G, dG, cb, arena, launches);
     return;
   }
   // key includes the buffer addresses baked into the captured launches
-  td_ctx::GraphKey key{static_cast<int>(G.n) | static_cast<int>(G.cb << 8),
+  td_ctx::GraphKey key{static_cast<int>(G.n) | static_cast<int>(G.cb << 8) | (slot << 16),
                        static_cast<int>(G.lb), ctx->dp.tiles_per_lane, ctx->dp.block};
   auto& p = ctx->dp_graph_ptrs[key];
   auto it = ctx->dp_graphs.find(key);
@@ -1065,32 +1066,39 @@ float Elapsed(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-void SolveOne(td_ctx* ctx, int n, const uint64_t* adj, uint8_t* td_out,
-              int8_t* parent_out, td_dp_stats* stats) {
-  const auto t0 = std::chrono::steady_clock::now();
-  ValidateGraph(n, adj, kMaxDPN);
-  // the N(H) chunk tables cover 28 high vertices
+// Record of one graph for the single-graph level kernels: the context's low
+// bits (more where the N(H) chunk tables, which cover 28 high vertices, need
+// it) and its class layout where the graph is large enough.
+DPGraph SingleGraph(td_ctx* ctx, int n, const uint64_t* adj) {
   const int lb = std::max(ctx->dp.low_bits, n - 28);
   const int nh = n > lb ? n - lb : 0;
   const int cbits = (ctx->dp.class_bits > 0 && nh >= ctx->dp.class_min_nh &&
                      nh > ctx->dp.class_bits)
                         ? ctx->dp.class_bits
                         : 0;
-  DPGraph G = MakeGraph(n, adj, lb, 0, cbits);
+  return MakeGraph(n, adj, lb, 0, cbits);
+}
+
+// descriptor, walk start set and step count in one device block, uploaded
+// from page-locked staging in one copy; the tree (64 parents, then the depth)
+// comes back the same way
+constexpr size_t kStartOff = (sizeof(DPGraph) + 15) / 16 * 16;
+constexpr size_t kInBytes = kStartOff + 16;
+constexpr size_t kTreeBytes = 64 + 16;
+
+void SolveOne(td_ctx* ctx, int n, const uint64_t* adj, uint8_t* td_out,
+              int8_t* parent_out, td_dp_stats* stats) {
+  const auto t0 = std::chrono::steady_clock::now();
+  ValidateGraph(n, adj, kMaxDPN);
+  DPGraph G = SingleGraph(ctx, n, adj);
   const uint64_t tbytes = TableBytes(G);
   uint8_t* arena = static_cast<uint8_t*>(ctx->Buffer("dp.arena", tbytes));
-  // descriptor, walk start set and step count in one device block, uploaded
-  // from page-locked staging in one copy; the walk's order comes back the
-  // same way
-  constexpr size_t kStartOff = (sizeof(DPGraph) + 15) / 16 * 16;
-  constexpr size_t kInBytes = kStartOff + 16;
   uint8_t* din = static_cast<uint8_t*>(ctx->Buffer("dp.graph", kInBytes));
   DPGraph* dG = reinterpret_cast<DPGraph*>(din);
   uint32_t* dstart = reinterpret_cast<uint32_t*>(din + kStartOff);
   const uint32_t* cb = PascalDevice(ctx);
-  // tree output: 64 parents then the depth
-  uint8_t* dtree = static_cast<uint8_t*>(ctx->Buffer("dp.tree_out", 64 + 16));
-  uint8_t* stage = static_cast<uint8_t*>(ctx->Pinned(kInBytes + 64 + 16));
+  uint8_t* dtree = static_cast<uint8_t*>(ctx->Buffer("dp.tree_out", kTreeBytes));
+  uint8_t* stage = static_cast<uint8_t*>(ctx->Pinned(kInBytes + kTreeBytes));
   const uint32_t full = n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
   std::memcpy(stage, &G, sizeof(G));
   std::memcpy(stage + kStartOff, &full, 4);
@@ -1131,6 +1139,77 @@ void SolveOne(td_ctx* ctx, int n, const uint64_t* adj, uint8_t* td_out,
 // ---------------------------------------------------------------------------
 // batch
 // ---------------------------------------------------------------------------
+// Graphs above kMaxBatchN vertices in a batch: one after another through
+// the single-graph level kernels on the context's stream, each graph's
+// decomposition on a second stream while the next graph's levels run.  Two
+// slots of (table, descriptor, tree output, staging) alternate; a slot is
+// reused only after the host has waited for its previous tree.  Adds the
+// level-kernel time, tree time (end of levels to end of tree), launches and
+// level-kernel count of these graphs to the outputs.
+void SolveLarge(td_ctx* ctx, const std::vector<int>& idx, const int32_t* ns,
+                const uint64_t* adj_concat, const std::vector<size_t>& row_off,
+                uint8_t* td_out, int8_t* parent_concat, double* level_ms, double* tree_ms,
+                uint64_t* launches, uint64_t* level_kernels) {
+  if (idx.empty()) return;
+  if (!ctx->side) TD_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+  for (auto& e : ctx->pev)
+    if (!e) TD_HIP(hipEventCreate(&e));
+  // every buffer at its final size before the first launch
+  uint64_t tmax = 0;
+  for (int i : idx) tmax = std::max(tmax, TableBytes(SingleGraph(ctx, ns[i], adj_concat + row_off[i])));
+  uint8_t* arena[2] = {static_cast<uint8_t*>(ctx->Buffer("dp.arena", tmax)),
+                       static_cast<uint8_t*>(ctx->Buffer("dp.arena1", tmax))};
+  uint8_t* din[2] = {static_cast<uint8_t*>(ctx->Buffer("dp.graph", kInBytes)),
+                     static_cast<uint8_t*>(ctx->Buffer("dp.graph1", kInBytes))};
+  uint8_t* dtree[2] = {static_cast<uint8_t*>(ctx->Buffer("dp.tree_out", kTreeBytes)),
+                       static_cast<uint8_t*>(ctx->Buffer("dp.tree_out1", kTreeBytes))};
+  const uint32_t* cb = PascalDevice(ctx);
+  constexpr size_t kSlot = kInBytes + kTreeBytes;
+  uint8_t* stage = static_cast<uint8_t*>(ctx->Pinned(2 * kSlot));
+  hipEvent_t* lv_start = ctx->pev;     // [slot]
+  hipEvent_t* lv_end = ctx->pev + 2;   // [slot]
+  hipEvent_t* tree_end = ctx->pev + 4; // [slot]
+  // results of the graph in slot s: its tree came back into the staging
+  auto consume = [&](int j) {
+    const int s = j & 1, i = idx[j];
+    TD_HIP(hipEventSynchronize(tree_end[s]));
+    const uint8_t* staged = stage + s * kSlot + kInBytes;
+    int32_t td = 0;
+    std::memcpy(&td, staged + 64, 4);
+    if (td_out) td_out[i] = static_cast<uint8_t>(td);
+    if (parent_concat) std::memcpy(parent_concat + row_off[i], staged, ns[i]);
+    *level_ms += Elapsed(lv_start[s], lv_end[s]);
+    *tree_ms += Elapsed(lv_end[s], tree_end[s]);
+  };
+  const int L = static_cast<int>(idx.size());
+  for (int j = 0; j < L; ++j) {
+    const int s = j & 1, i = idx[j];
+    if (j >= 2) consume(j - 2);  // frees slot s: its tree, copies and levels are done
+    const DPGraph G = SingleGraph(ctx, ns[i], adj_concat + row_off[i]);
+    uint8_t* in = stage + s * kSlot;
+    const uint32_t full = G.n >= 32 ? 0xFFFFFFFFu : ((1u << G.n) - 1u);
+    std::memcpy(in, &G, sizeof(G));
+    std::memcpy(in + kStartOff, &full, 4);
+    TD_HIP(hipMemcpyAsync(din[s], in, kInBytes, hipMemcpyHostToDevice, ctx->stream));
+    const DPGraph* dG = reinterpret_cast<const DPGraph*>(din[s]);
+    TD_HIP(hipEventRecord(lv_start[s], ctx->stream));
+    RunLevels(ctx, G, dG, cb, arena[s], launches, s);
+    TD_HIP(hipEventRecord(lv_end[s], ctx->stream));
+    *level_kernels += LevelKernelCount(G);
+    TD_HIP(hipStreamWaitEvent(ctx->side, lv_end[s], 0));
+    hipLaunchKernelGGL(DPTreeKernel, dim3(1), dim3(64), 0, ctx->side, dG, cb, arena[s],
+                       reinterpret_cast<const uint32_t*>(din[s] + kStartOff),
+                       reinterpret_cast<int8_t*>(dtree[s]),
+                       reinterpret_cast<int32_t*>(dtree[s] + 64));
+    TD_HIP(hipGetLastError());
+    ++*launches;
+    TD_HIP(hipMemcpyAsync(stage + s * kSlot + kInBytes, dtree[s], 64 + 4,
+                          hipMemcpyDeviceToHost, ctx->side));
+    TD_HIP(hipEventRecord(tree_end[s], ctx->side));
+  }
+  for (int j = std::max(0, L - 2); j < L; ++j) consume(j);
+}
+
 template <int LB>
 void LaunchBatch(td_ctx* ctx, int mode, int ngraphs,
                  const DPGraph* Gs, const DPGraph* dGs,
@@ -1230,17 +1309,50 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
   const int lb = ctx->dp.low_bits;
   std::vector<size_t> row_off(ngraphs + 1, 0);
   for (int i = 0; i < ngraphs; ++i) {
-    TD_REQUIRE(ns[i] >= 1 && ns[i] <= kMaxBatchN, TD_E_INVALID,
-               "batch graphs need n in [1, 26]");
+    TD_REQUIRE(ns[i] >= 1 && ns[i] <= kMaxDPN, TD_E_INVALID,
+               "batch graphs need n in [1, 32]");
     row_off[i + 1] = row_off[i] + ns[i];
   }
+  for (int i = 0; i < ngraphs; ++i) ValidateGraph(ns[i], adj_concat + row_off[i], kMaxDPN);
+  // graphs above kMaxBatchN vertices go through the single-graph kernels
+  std::vector<int> large;
   for (int i = 0; i < ngraphs; ++i)
-    ValidateGraph(ns[i], adj_concat + row_off[i], kMaxBatchN);
-  // largest graphs first so the persistent queue ends with short work
-  std::vector<int> perm(ngraphs);
-  for (int i = 0; i < ngraphs; ++i) perm[i] = i;
+    if (ns[i] > kMaxBatchN) large.push_back(i);
+  uint64_t subsets = 0, alg = 0;
+  for (int i = 0; i < ngraphs; ++i) {
+    subsets += 1ull << ns[i];
+    alg += (1ull << (ns[i] - 1)) * static_cast<uint64_t>(ns[i] + 2);
+  }
+  double level_ms = 0.0, tree_ms = 0.0;
+  uint64_t launches = 0, level_kernels = 0;
+  SolveLarge(ctx, large, ns, adj_concat, row_off, td_out, parent_concat, &level_ms, &tree_ms,
+             &launches, &level_kernels);
+  ctx->have_dp = false;  // queries refer to single-graph solves only
+  const int total_graphs = ngraphs;
+  auto finish = [&] {
+    if (!stats) return;
+    stats->graphs = static_cast<uint64_t>(total_graphs);
+    stats->level_kernels = level_kernels;
+    stats->subsets = subsets;
+    stats->alg_bytes = alg;
+    stats->launches = launches;
+    stats->level_ms = level_ms;
+    stats->tree_ms = tree_ms;
+    stats->total_ms = std::chrono::duration<double, std::milli>(
+                          std::chrono::steady_clock::now() - t0)
+                          .count();
+  };
+  // the rest: largest first so the persistent queue ends with short work
+  std::vector<int> perm;
+  for (int i = 0; i < ngraphs; ++i)
+    if (ns[i] <= kMaxBatchN) perm.push_back(i);
   std::stable_sort(perm.begin(), perm.end(),
                    [&](int a, int b) { return ns[a] > ns[b]; });
+  if (perm.empty()) {
+    finish();
+    return;
+  }
+  ngraphs = static_cast<int>(perm.size());
   // the graph records go up from page-locked memory; they are built on
   // several host threads (about 1 us each), then given their arena offsets
   DPGraph* Gs = static_cast<DPGraph*>(ctx->Pinned(sizeof(DPGraph) * ngraphs));
@@ -1278,50 +1390,36 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
   TD_HIP(hipMemcpyAsync(dGs, Gs, sizeof(DPGraph) * ngraphs,
                         hipMemcpyHostToDevice, ctx->stream));
   TD_HIP(hipMemcpyAsync(dstart, start.data(), 4 * ngraphs, hipMemcpyHostToDevice, ctx->stream));
-  uint64_t launches = 0;
+  uint64_t small_launches = 0;
   TD_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
   switch (lb) {
-    case 2: LaunchBatch<2>(ctx, mode, ngraphs, Gs, dGs, cb, arena, dq, &launches); break;
-    case 3: LaunchBatch<3>(ctx, mode, ngraphs, Gs, dGs, cb, arena, dq, &launches); break;
-    case 4: LaunchBatch<4>(ctx, mode, ngraphs, Gs, dGs, cb, arena, dq, &launches); break;
-    case 5: LaunchBatch<5>(ctx, mode, ngraphs, Gs, dGs, cb, arena, dq, &launches); break;
+    case 2: LaunchBatch<2>(ctx, mode, ngraphs, Gs, dGs, cb, arena, dq, &small_launches); break;
+    case 3: LaunchBatch<3>(ctx, mode, ngraphs, Gs, dGs, cb, arena, dq, &small_launches); break;
+    case 4: LaunchBatch<4>(ctx, mode, ngraphs, Gs, dGs, cb, arena, dq, &small_launches); break;
+    case 5: LaunchBatch<5>(ctx, mode, ngraphs, Gs, dGs, cb, arena, dq, &small_launches); break;
     default: throw Error{TD_E_INVALID, "low_bits must be in [2, 5]"};
   }
   TD_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
   // mode 0: the persistent kernel (its queue reset aside); else the launches
-  const uint64_t level_kernels = mode == 0 ? 1 : launches;
+  level_kernels += mode == 0 ? 1 : small_launches;
   hipLaunchKernelGGL(DPTreeKernel, dim3(ngraphs), dim3(64), 0, ctx->stream,
                      dGs, cb, arena, dstart, dpar, dtd);
   TD_HIP(hipGetLastError());
-  ++launches;
+  launches += small_launches + 1;
   TD_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
   std::vector<int8_t> par(64ull * ngraphs);
   std::vector<int32_t> tds(ngraphs);
   TD_HIP(hipMemcpyAsync(par.data(), dpar, par.size(), hipMemcpyDeviceToHost, ctx->stream));
   TD_HIP(hipMemcpyAsync(tds.data(), dtd, 4ull * ngraphs, hipMemcpyDeviceToHost, ctx->stream));
   TD_HIP(hipStreamSynchronize(ctx->stream));
-  uint64_t subsets = 0, alg = 0;
   for (int j = 0; j < ngraphs; ++j) {
     const int i = perm[j];
-    const int n = ns[i];
     if (td_out) td_out[i] = static_cast<uint8_t>(tds[j]);
-    if (parent_concat) std::memcpy(parent_concat + row_off[i], par.data() + 64ull * j, n);
-    subsets += 1ull << n;
-    alg += (1ull << (n - 1)) * static_cast<uint64_t>(n + 2);
+    if (parent_concat) std::memcpy(parent_concat + row_off[i], par.data() + 64ull * j, ns[i]);
   }
-  ctx->have_dp = false;  // queries refer to single-graph solves only
-  if (stats) {
-    stats->graphs = static_cast<uint64_t>(ngraphs);
-    stats->level_kernels = level_kernels;
-    stats->subsets = subsets;
-    stats->alg_bytes = alg;
-    stats->launches = launches;
-    stats->level_ms = Elapsed(ctx->ev[0], ctx->ev[1]);
-    stats->tree_ms = Elapsed(ctx->ev[1], ctx->ev[2]);
-    stats->total_ms = std::chrono::duration<double, std::milli>(
-                          std::chrono::steady_clock::now() - t0)
-                          .count();
-  }
+  level_ms += Elapsed(ctx->ev[0], ctx->ev[1]);
+  tree_ms += Elapsed(ctx->ev[1], ctx->ev[2]);
+  finish();
 }
 
 // ---------------------------------------------------------------------------

@@ -105,12 +105,17 @@ int td_dp_get_elimination(td_ctx* ctx, int32_t nverts, int32_t subset_size,
                           uint64_t subset_code, int32_t* order_out);
 
 /* Many graphs in one call (no reference counterpart: the reference loops
- * over input files one at a time, Treedepth/main.cpp:130).  n[i] <= 26;
+ * over input files one at a time, Treedepth/main.cpp:130).  n[i] <= 32;
  * adj_concat holds the rows of graph 0, then graph 1, ...; parent_concat
- * likewise (sum n[i] entries).  mode: 0 = one persistent launch (graphs
- * pulled from a device queue, all levels inside), 1 = level-synchronous
- * launches over the whole batch, 2 = one launch per graph and level (the
- * baseline the persistent launch is measured against). */
+ * likewise (sum n[i] entries).  For the graphs of at most 26 vertices, mode:
+ * 0 = one persistent launch (graphs pulled from a device queue, all levels
+ * inside), 1 = level-synchronous launches over the whole batch, 2 = one
+ * launch per graph and level (the baseline the persistent launch is
+ * measured against).  Larger graphs go one after another through
+ * td_dp_solve's level kernels, each graph's decomposition on a second stream
+ * while the next graph's levels run (two tables of the largest such graph
+ * are held).  Results equal td_dp_solve's for every graph; afterwards the
+ * td_dp_get_* queries return TD_E_STATE until the next td_dp_solve. */
 int td_dp_solve_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
                       const uint64_t* adj_concat, uint8_t* treedepth_out,
                       int8_t* parent_concat, int32_t mode,

@@ -285,6 +285,47 @@ def test_batch_modes_equal_single(ctx, oracle):
         assert st.graphs == len(graphs)
 
 
+@pytest.mark.parametrize("use_graph", [1, 0])
+def test_batch_with_large_graphs_equals_single(ctx, oracle, use_graph):
+    # graphs above 26 vertices take the two-slot pipeline (levels on the
+    # context's stream, trees on a second one); five of them reuse each slot
+    rng = random.Random(11)
+    graphs = [G.gnp(27, 0.2, 1), G.gnp(10, 0.4, 2), G.grid_dropped(4, 7, 0.1, 3),
+              G.gnp(29, 0.15, 4), G.gnp(15, 0.3, 5), G.gnp(27, 0.25, 6), G.gnp(20, 0.2, 7),
+              G.gnp(28, 0.18, 8)]
+    graphs += [G.gnp(rng.randint(6, 16), 0.3, rng.getrandbits(32)) for _ in range(5)]
+    ctx.set_param("dp.use_graph", use_graph)
+    try:
+        print("batch with large graphs, use_graph", use_graph, flush=True)
+        dp = api.DynamicGPU(ctx)
+        want, kernels = [], 0
+        for g in graphs:
+            dp(g)
+            want.append((dp.treedepth, dp.parent))
+            kernels += dp.stats.level_kernels if g.n > 26 else 0
+        tds, pars, st = api.dp_solve_batch(ctx, graphs, 0)
+        assert list(zip(tds, pars)) == want
+        assert st.graphs == len(graphs)
+        assert st.subsets == sum(1 << g.n for g in graphs)
+        assert st.level_kernels == kernels + 1  # plus the persistent launch
+        assert st.level_ms > 0 and st.tree_ms > 0
+        for g, (td, par) in zip(graphs, want):
+            if g.n <= 20:
+                assert (td, par) == oracle.dp_fast(g)
+            else:
+                assert oracle.certify(g, par, td)
+        # only large graphs: no persistent launch at all
+        big = [g for g in graphs if g.n > 26]
+        tds2, pars2, st2 = api.dp_solve_batch(ctx, big, 0)
+        assert list(zip(tds2, pars2)) == [w for g, w in zip(graphs, want) if g.n > 26]
+        assert st2.level_kernels == kernels
+        with pytest.raises(api.TDError) as e:
+            dp.level(3)
+        assert e.value.code == api.TD_E_STATE
+    finally:
+        ctx.set_param("dp.use_graph", 1)
+
+
 def test_graph_capture_replay_equal(ctx, oracle):
     dp = api.DynamicGPU(ctx)
     gs = [G.gnp(22, 0.3, s) for s in range(3)]
