@@ -1182,32 +1182,39 @@ void SolveLarge(td_ctx* ctx, const std::vector<int>& idx, const int32_t* ns,
     *tree_ms += Elapsed(lv_end[s], tree_end[s]);
   };
   const int L = static_cast<int>(idx.size());
-  for (int j = 0; j < L; ++j) {
-    const int s = j & 1, i = idx[j];
-    if (j >= 2) consume(j - 2);  // frees slot s: its tree, copies and levels are done
-    const DPGraph G = SingleGraph(ctx, ns[i], adj_concat + row_off[i]);
-    uint8_t* in = stage + s * kSlot;
-    const uint32_t full = G.n >= 32 ? 0xFFFFFFFFu : ((1u << G.n) - 1u);
-    std::memcpy(in, &G, sizeof(G));
-    std::memcpy(in + kStartOff, &full, 4);
-    TD_HIP(hipMemcpyAsync(din[s], in, kInBytes, hipMemcpyHostToDevice, ctx->stream));
-    const DPGraph* dG = reinterpret_cast<const DPGraph*>(din[s]);
-    TD_HIP(hipEventRecord(lv_start[s], ctx->stream));
-    RunLevels(ctx, G, dG, cb, arena[s], launches, s);
-    TD_HIP(hipEventRecord(lv_end[s], ctx->stream));
-    *level_kernels += LevelKernelCount(G);
-    TD_HIP(hipStreamWaitEvent(ctx->side, lv_end[s], 0));
-    hipLaunchKernelGGL(DPTreeKernel, dim3(1), dim3(64), 0, ctx->side, dG, cb, arena[s],
-                       reinterpret_cast<const uint32_t*>(din[s] + kStartOff),
-                       reinterpret_cast<int8_t*>(dtree[s]),
-                       reinterpret_cast<int32_t*>(dtree[s] + 64));
-    TD_HIP(hipGetLastError());
-    ++*launches;
-    TD_HIP(hipMemcpyAsync(stage + s * kSlot + kInBytes, dtree[s], 64 + 4,
-                          hipMemcpyDeviceToHost, ctx->side));
-    TD_HIP(hipEventRecord(tree_end[s], ctx->side));
+  try {
+    for (int j = 0; j < L; ++j) {
+      const int s = j & 1, i = idx[j];
+      if (j >= 2) consume(j - 2);  // frees slot s: its tree, copies and levels are done
+      const DPGraph G = SingleGraph(ctx, ns[i], adj_concat + row_off[i]);
+      uint8_t* in = stage + s * kSlot;
+      const uint32_t full = G.n >= 32 ? 0xFFFFFFFFu : ((1u << G.n) - 1u);
+      std::memcpy(in, &G, sizeof(G));
+      std::memcpy(in + kStartOff, &full, 4);
+      TD_HIP(hipMemcpyAsync(din[s], in, kInBytes, hipMemcpyHostToDevice, ctx->stream));
+      const DPGraph* dG = reinterpret_cast<const DPGraph*>(din[s]);
+      TD_HIP(hipEventRecord(lv_start[s], ctx->stream));
+      RunLevels(ctx, G, dG, cb, arena[s], launches, s);
+      TD_HIP(hipEventRecord(lv_end[s], ctx->stream));
+      *level_kernels += LevelKernelCount(G);
+      TD_HIP(hipStreamWaitEvent(ctx->side, lv_end[s], 0));
+      hipLaunchKernelGGL(DPTreeKernel, dim3(1), dim3(64), 0, ctx->side, dG, cb, arena[s],
+                         reinterpret_cast<const uint32_t*>(din[s] + kStartOff),
+                         reinterpret_cast<int8_t*>(dtree[s]),
+                         reinterpret_cast<int32_t*>(dtree[s] + 64));
+      TD_HIP(hipGetLastError());
+      ++*launches;
+      TD_HIP(hipMemcpyAsync(stage + s * kSlot + kInBytes, dtree[s], 64 + 4,
+                            hipMemcpyDeviceToHost, ctx->side));
+      TD_HIP(hipEventRecord(tree_end[s], ctx->side));
+    }
+    for (int j = std::max(0, L - 2); j < L; ++j) consume(j);
+  } catch (...) {
+    // nothing queued may still use the slots when the error reaches the caller
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->side);
+    throw;
   }
-  for (int j = std::max(0, L - 2); j < L; ++j) consume(j);
 }
 
 template <int LB>
