@@ -326,6 +326,23 @@ def test_batch_with_large_graphs_equals_single(ctx, oracle, use_graph):
         ctx.set_param("dp.use_graph", 1)
 
 
+def test_batch_max_size_graphs_equal_single(ctx, oracle):
+    # n = 32 and 31 through the batch pipeline (4 GiB tables in each slot),
+    # three graphs so one slot is reused
+    graphs = [G.gnp(32, 0.15, 21), G.gnp(31, 0.2, 4), G.grid(4, 8)]
+    print("batch max size", flush=True)
+    dp = api.DynamicGPU(ctx)
+    want = []
+    for g in graphs:
+        dp(g)
+        want.append((dp.treedepth, dp.parent))
+    tds, pars, st = api.dp_solve_batch(ctx, graphs, 0)
+    assert list(zip(tds, pars)) == want
+    assert st.subsets == (1 << 32) * 2 + (1 << 31)
+    for g, (td, par) in zip(graphs, want):
+        assert oracle.certify(g, par, td)
+
+
 def test_graph_capture_replay_equal(ctx, oracle):
     dp = api.DynamicGPU(ctx)
     gs = [G.gnp(22, 0.3, s) for s in range(3)]
