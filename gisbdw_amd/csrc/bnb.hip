@@ -152,49 +152,52 @@ __device__ __forceinline__ int CeilLog2(int x) {  // x >= 1
 // For a connected C with |E(C)| = |C| - 1 the BFS tree is C: the result is
 // td(C) and *top an optimal root.  For any other connected C the BFS tree is
 // a spanning subgraph, so the result is a lower bound on td(C).
-// sL / sPar: this wave's LDS scratch (64 entries each).
-__device__ int TreeRank(uint64_t myadj, uint64_t C, int* top, uint64_t* sL, int8_t* sPar) {
+// sU / sD: this wave's LDS scratch (64 entries each) for U and D; a child
+// adds its L to its parent's entries with LDS atomics (the value an OR
+// returns shows the bits an earlier child already brought, which go to D).
+__device__ int TreeRank(uint64_t myadj, uint64_t C, int* top, uint64_t* sU, uint64_t* sD) {
   const int lane = threadIdx.x & 63;
   const bool in = (C >> lane) & 1ull;
   const uint64_t root = C & (0ull - C);
   int mylvl = (root >> lane) & 1ull ? 0 : -1;
+  int mypar = -1;
   uint64_t seen = root, front = root;
   int depth = 0;
-  sPar[lane] = -1;
+  sU[lane] = 0;
+  sD[lane] = 0;
   for (;;) {
     const uint64_t nb = Ballot(in && !((seen >> lane) & 1ull) && (myadj & front) != 0ull);
     if (!nb) break;
     ++depth;
     if ((nb >> lane) & 1ull) {
       mylvl = depth;
-      sPar[lane] = static_cast<int8_t>(__builtin_ctzll(myadj & front));
+      mypar = __builtin_ctzll(myadj & front);
     }
     seen |= nb;
     front = nb;
   }
   Sync();
   int rank = 0;
+  uint64_t myL = 0;
   for (int l = depth; l >= 0; --l) {
-    const uint64_t below = Ballot(mylvl == l + 1);
+    if (mylvl == l + 1) {
+      const uint64_t old = atomicOr(reinterpret_cast<unsigned long long*>(sU + mypar),
+                                    static_cast<unsigned long long>(myL));
+      if (old & myL)
+        atomicOr(reinterpret_cast<unsigned long long*>(sD + mypar),
+                 static_cast<unsigned long long>(old & myL));
+    }
+    Sync();
     if (mylvl == l) {
-      uint64_t U = 0, dup = 0;
-      for (uint64_t c = myadj & below; c; c &= c - 1) {
-        const int u = __builtin_ctzll(c);
-        if (sPar[u] == lane) {
-          const uint64_t Lu = sL[u];
-          dup |= U & Lu;
-          U |= Lu;
-        }
-      }
+      const uint64_t U = sU[lane], dup = sD[lane];
       const int m = dup ? 63 - __builtin_clzll(dup) : -1;  // highest duplicated bit
       const uint64_t above = m < 0 ? ~0ull : (m >= 63 ? 0ull : ~((2ull << m) - 1ull));
       const uint64_t freeb = ~U & above;
       const int b = freeb ? __builtin_ctzll(freeb) : 63;
       rank = b + 1;
       const uint64_t higher = b >= 63 ? 0ull : (U & ~((2ull << b) - 1ull));
-      sL[lane] = (1ull << b) | higher;
+      myL = (1ull << b) | higher;
     }
-    Sync();
   }
   // largest rank (<= 64, 7 bits) by ballots, and its (unique) vertex
   uint64_t M = C;
@@ -399,21 +402,33 @@ __device__ __forceinline__ int PickRoot(bool use_scores, float score,
 // skipped.  Domination implies deg_C(u) <= deg_C(v) with equality only for
 // twins; u is skipped only for a dominator of higher degree, or of equal
 // degree and lower id, so every chain of skips ends at a root that is tried.
-__device__ __forceinline__ uint64_t Undominated(uint64_t myadj, uint64_t C) {
+// A dominator of u is u's lowest neighbour w in C or a neighbour of w (w is
+// in N_C(u) \ {v}, hence in N_C(v), unless v = w), so each lane tests only
+// those, reading their rows from the graph's copy in LDS.
+__device__ __forceinline__ uint64_t Undominated(uint64_t myadj, const uint64_t* adj,
+                                                uint64_t C) {
   const int lane = threadIdx.x & 63;
   const bool in = (C >> lane) & 1ull;
   const uint64_t Nu = myadj & C;
   const int du = __builtin_popcountll(Nu);
+  uint64_t cand = 0;
+  if (in) {
+    if (Nu) {
+      const int w = __builtin_ctzll(Nu);
+      cand = (adj[w] & C) | (1ull << w);
+    } else {
+      cand = C;
+    }
+    cand &= ~(1ull << lane);
+  }
   bool dominated = false;
-  for (uint64_t m = C; m; m &= m - 1) {
+  for (uint64_t m = cand; m && !dominated; m &= m - 1) {
     const int v = __builtin_ctzll(m);
-    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(myadj), v);
-    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(myadj >> 32), v);
-    const uint64_t Nv = ((static_cast<uint64_t>(hi) << 32) | lo) & C;
+    const uint64_t Nv = adj[v] & C;
     const int dv = __builtin_popcountll(Nv);
     const bool sub = ((Nu & ~(1ull << v)) & ~(Nv & ~(1ull << lane))) == 0ull;
     const bool higher = dv > du || (dv == du && v < lane);
-    dominated |= in && v != lane && sub && higher;
+    dominated = sub && higher;
   }
   return C & ~Ballot(dominated);
 }
@@ -421,14 +436,15 @@ __device__ __forceinline__ uint64_t Undominated(uint64_t myadj, uint64_t C) {
 struct Search {
   Frame* st;  // LDS stack
   uint64_t myadj;
+  const uint64_t* adj;  // the graph's rows in LDS (shared by the workgroup)
   float score;
   bool use_scores;
   Memo memo;
   uint64_t nodes, budget, hits;
   bool aborted;          // node budget exhausted: results are not proofs
   const int* cancel;     // LDS flag set when another wave of the graph succeeded
-  uint64_t* tl;          // TreeRank scratch (this wave's LDS)
-  int8_t* tp;
+  uint64_t* tu;          // TreeRank scratch (this wave's LDS): unions
+  uint64_t* tdup;        // and duplicated ranks per parent
   bool building;         // inside Build: every decided set must leave a root
 };
 
@@ -499,7 +515,7 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
         }
         // exact treedepth and an optimal root, no search
         int top;
-        const int t = TreeRank(S.myadj, C, &top, S.tl, S.tp);
+        const int t = TreeRank(S.myadj, C, &top, S.tu, S.tdup);
         int slot;
         const uint32_t md = MemoResolve(S.memo, C, me, &slot);
         MemoStore(S.memo, C, slot, md, static_cast<uint32_t>(t), static_cast<uint32_t>(t),
@@ -515,7 +531,7 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
       // + 1, so its ranking can only exceed k when k is below both
       if (lb <= k && ecount - (n - 1) <= 12 && k < 32 - __builtin_clz(static_cast<uint32_t>(n)) &&
           k <= height)
-        lb = max(lb, TreeRank(S.myadj, C, nullptr, S.tl, S.tp));
+        lb = max(lb, TreeRank(S.myadj, C, nullptr, S.tu, S.tdup));
       int slot;
       const uint32_t md = MemoResolve(S.memo, C, me, &slot);
       if (slot >= 0) {
@@ -529,7 +545,7 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
         --sp;
         continue;
       }
-      const uint64_t roots = n >= 4 ? Undominated(S.myadj, C) : C;
+      const uint64_t roots = n >= 4 ? Undominated(S.myadj, S.adj, C) : C;
       if (lane == 0) {
         F->cand = roots;
         F->state = kNextRoot;
@@ -591,7 +607,7 @@ __device__ bool Feasible(Search& S, uint64_t C0, int k0) {
           break;
         }
         if (ed == nd - 1) {  // a tree: decided exactly here
-          if (TreeRank(S.myadj, Dc, nullptr, S.tl, S.tp) > k - 1) {
+          if (TreeRank(S.myadj, Dc, nullptr, S.tu, S.tdup) > k - 1) {
             reject = true;
             break;
           }
@@ -816,10 +832,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
   __shared__ uint64_t s_bst[130];
   __shared__ int s_bk[130], s_bpar[130];
   __shared__ uint64_t s_comps[64];
+  __shared__ uint64_t s_adj[64];
   __shared__ int s_gi, s_ub, s_lb, s_heur, s_ncomp;
   __shared__ int s_found, s_winner, s_wroot, s_abort;
-  __shared__ uint64_t s_tl[kMaxWaves][64];
-  __shared__ int8_t s_tp[kMaxWaves][64];
+  __shared__ uint64_t s_tu[kMaxWaves][64];
+  __shared__ uint64_t s_tdup[kMaxWaves][64];
   __shared__ unsigned long long s_nodes, s_hits;
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
@@ -827,8 +844,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
   Search S;
   S.building = false;
   S.st = s_stack[wid];
-  S.tl = s_tl[wid];
-  S.tp = s_tp[wid];
+  S.tu = s_tu[wid];
+  S.tdup = s_tdup[wid];
   S.memo.table =
       memo_base + (static_cast<size_t>(blockIdx.x * W + wid) << (memo_log2 + 2));
   S.memo.mask = (1u << memo_log2) - 1u;
@@ -841,6 +858,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
     const int n = static_cast<int>(g->n);
     const uint64_t V = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
     S.myadj = lane < n ? g->adj[lane] : 0ull;
+    // every wave has left the previous graph (barrier above)
+    if (wid == 0) s_adj[lane] = S.myadj;
+    __syncthreads();
+    S.adj = s_adj;
     S.score = lane < n ? g->score[lane] : 0.f;
     S.use_scores = g->use_scores != 0;
     S.memo.tag = tag_base + static_cast<uint32_t>(gi) + 1u;
@@ -938,7 +959,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
         const bool lb_fail = LowerBound(S.myadj, in_c, C, k) > k;
         if (!lb_fail) {
           // this wave's share of the top-level roots, in key order
-          uint64_t cand = nc >= 4 ? Undominated(S.myadj, C) : C;
+          uint64_t cand = nc >= 4 ? Undominated(S.myadj, S.adj, C) : C;
           for (int i = 0; cand; ++i) {
             const int v = PickRoot(S.use_scores, S.score, S.myadj, C, cand);
             cand &= ~(1ull << v);
