@@ -35,7 +35,10 @@ after td_dp_autotune_batch; and the oracle's CPU paths for the same
 configurations (faithful DynamicCPU, flat-table DP, BranchAndBound) on
 bounded samples.
 
-Other workloads: --workload dp20 | dpbatch | bottomup.
+Other workloads: --workload dp20 | dpbatch | bottomup | bnb.  bnb: the
+1024-graph sparse workload (n in [40,64]), learned-ordered against plain
+branch-and-bound at 5e4 nodes per graph and 8 waves per graph, the proposal
+trained first on the GPU; value = certified graphs/s of the learned search.
 """
 from __future__ import annotations
 
@@ -58,7 +61,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="dp30", choices=["dp30", "dp20", "dpbatch", "bottomup"])
+    ap.add_argument("--workload", default="dp30",
+                    choices=["dp30", "dp20", "dpbatch", "bottomup", "bnb"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--batch-mode", type=int, default=0)
     ap.add_argument("--low-bits", type=int, default=None)
@@ -336,6 +340,31 @@ def cpu_extra(threads):
     return out
 
 
+def cpu_baseline_bnb(threads):
+    """The oracle's BranchAndBound restatement (EdgeLowerBound, the
+    HighestDegree(SpanningTree(Variance)) seed chain, branch_and_bound.cpp:
+    9-115), graph-parallel on every 16th graph of the 1024-graph workload with
+    2e6 search nodes per graph."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib
+    from gisbdw_amd import graphs as G
+
+    o = oracle_lib.Oracle()
+    sample = G.workload_bnb(1024, 1000)[::16]
+    budget = 2_000_000
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        res = list(ex.map(lambda h: o.bnb(h, 0, 1, budget), sample))
+    dt = time.perf_counter() - t0
+    cert = sum(1 for r in res if r[3])
+    return {"value": cert / dt, "unit": "certified graphs/s", "cores": threads, "kind": "port",
+            "sample": f"oracle BranchAndBound restatement on {len(sample)} of the 1024 graphs "
+                      f"(every 16th), {budget} nodes per graph: {cert} certified in {dt:.2f} s, "
+                      f"{sum(r[4] for r in res) / dt:.3g} nodes/s"}
+
+
 def cpu_baseline_bottom_up(g):
     """The oracle's BottomUpHeuristicGPU restatement (union-find arrays per
     subset, bottom_up_heuristic_gpu_algorithm.cu:57-95) on one core, over the
@@ -552,6 +581,71 @@ def main():
                          "timing": "wall clock of the whole call"},
             "extra": {"treedepth": bu.treedepth},
         }
+    elif args.workload == "bnb":
+        # 1024 sparse graphs, n in [40,64]: the proposal is trained on the GPU
+        # (DP labels, 6 epochs), then each step runs the plain search and the
+        # learned-ordered one (scoring included) on the whole batch
+        from gisbdw_amd import learn
+
+        graphs = G.workload_bnb(1024, 1000)
+        ctx.set_param("bnb.node_budget", 50000)
+        ctx.set_param("bnb.waves_per_graph", 8)
+        prop = learn.Proposal(ctx)
+        prop.init(1)
+        t0 = time.perf_counter()
+        ns, adj, tg, _ = learn.make_dataset(ctx, 800, 250, seed=11, nmin=8, nmax=24)
+        prop.train(ns, adj, tg, epochs=6, batch=512, lr=2e-3, seed=3)
+        train_s = time.perf_counter() - t0
+
+        def run(learned):
+            t = time.perf_counter()
+            sc = prop.scores(graphs) if learned else None
+            td, _, cert, st = api.bnb_solve_batch(ctx, graphs, scores=sc)
+            return time.perf_counter() - t, td, cert, st
+
+        for _ in range(args.warmup):
+            run(False)
+            run(True)
+        tot = {False: [0.0, 0, 0], True: [0.0, 0, 0]}
+        answers = {}
+        for i in range(args.steps):
+            for learned in (False, True):
+                dt, td, cert, st = run(learned)
+                a = tot[learned]
+                a[0] += dt
+                a[1] += int(sum(cert))
+                a[2] += st.nodes
+                answers[learned] = (td, cert)
+            print(f"[bench] step {i} certified plain {tot[False][1]} learned {tot[True][1]}",
+                  file=sys.stderr, flush=True)
+        agree = all(a == b for a, b, c, d in zip(answers[False][0], answers[True][0],
+                                                   answers[False][1], answers[True][1]) if c and d)
+        rate = {k: v[1] / v[0] for k, v in tot.items()}
+        out = {
+            "metric": "certified graphs/sec (learned-ordered B&B, 1024 graphs, n in [40,64])",
+            "value": rate[True],
+            "unit": "certified graphs/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": (tot[False][0] + tot[True][0]) / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+            "config": {"workload": "bnb-1024: 256 random cubic, 128 7x7 + 128 8x8 grids "
+                                   "relabelled, 512 random trees + 10% edges; seed 1000",
+                       "graphs": len(graphs), "node_budget_per_graph": 50000,
+                       "waves_per_graph": 8},
+            "roofline": None,
+            "extra": {
+                "roofline_note": "latency-bound search: no streaming roofline; nodes/s below",
+                "plain_certified_graphs_per_s": rate[False],
+                "learned_over_plain": rate[True] / max(rate[False], 1e-9),
+                "certified_per_step": {"plain": tot[False][1] // args.steps,
+                                       "learned": tot[True][1] // args.steps},
+                "nodes_per_s": {"plain": tot[False][2] / tot[False][0],
+                                "learned": tot[True][2] / tot[True][0]},
+                "certified_answers_agree": agree,
+                "proposal_training_s": train_s,
+            },
+        }
     else:
         graphs = G.workload_dp_batch(4096, 0)
         ns, adj = api.pack_graphs(graphs)  # inputs packed once, outside the timing
@@ -606,6 +700,9 @@ def main():
     if not args.no_cpu and args.workload == "bottomup":
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline_bottom_up(graphs[0])
+    elif not args.no_cpu and args.workload == "bnb":
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline_bnb(min(os.cpu_count() or 1, 16))
     elif not args.no_cpu:
         g0 = graphs[0]
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
