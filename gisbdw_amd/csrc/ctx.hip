@@ -67,18 +67,19 @@ void ValidateGraph(int n, const uint64_t* adj, int max_n) {
 
 }  // namespace tdhip
 
-void* td_ctx::Pinned(size_t bytes) {
-  if (pinned && pinned_bytes >= bytes) return pinned;
-  if (pinned) {
+void* td_ctx::Pinned(size_t bytes, int slot) {
+  void*& p = pinned[slot];
+  if (p && pinned_bytes[slot] >= bytes) return p;
+  if (p) {
     TD_HIP(hipStreamSynchronize(stream));
     if (side) TD_HIP(hipStreamSynchronize(side));
-    TD_HIP(hipHostFree(pinned));
-    pinned = nullptr;
-    pinned_bytes = 0;
+    TD_HIP(hipHostFree(p));
+    p = nullptr;
+    pinned_bytes[slot] = 0;
   }
-  TD_HIP(hipHostMalloc(&pinned, bytes, hipHostMallocDefault));
-  pinned_bytes = bytes;
-  return pinned;
+  TD_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+  pinned_bytes[slot] = bytes;
+  return p;
 }
 
 void* td_ctx::Buffer(const std::string& name, size_t bytes) {
@@ -198,7 +199,8 @@ void td_ctx_destroy(td_ctx* c) {
   for (auto& kv : c->dp_graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto& kv : c->bufs)
     if (kv.second.p) (void)hipFree(kv.second.p);
-  if (c->pinned) (void)hipHostFree(c->pinned);
+  for (void* p : c->pinned)
+    if (p) (void)hipHostFree(p);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : c->pev)

@@ -125,11 +125,13 @@ struct td_ctx {
   std::map<std::string, tdhip::DeviceBuffer> bufs;
   void* Buffer(const std::string& name, size_t bytes);
   void Release(const std::string& name);
-  // page-locked host staging for the small per-call copies (grown on
-  // demand; every call synchronizes before returning, so it is reused)
-  void* pinned = nullptr;
-  size_t pinned_bytes = 0;
-  void* Pinned(size_t bytes);
+  // page-locked host staging for the per-call copies, two independent
+  // buffers (grown on demand; every call synchronizes before returning, so
+  // they are reused): slot 0 for single-graph staging, slot 1 for the batch
+  // records and results, which live across the large-graph pipeline
+  void* pinned[2] = {nullptr, nullptr};
+  size_t pinned_bytes[2] = {0, 0};
+  void* Pinned(size_t bytes, int slot = 0);
 
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // second stream and events of the two-slot pipeline that overlaps one

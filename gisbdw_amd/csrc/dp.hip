@@ -19,6 +19,7 @@
 // of the same tile (registers).  One lane owns a tile; a lane walks
 // consecutive ranks with Gosper's successor (colex order = numeric order).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -523,6 +524,67 @@ __global__ __launch_bounds__(1024) void DPChunkKernel(
 // pre-sorted largest first) and resolve every level of a graph with a
 // workgroup barrier between levels, so no level of a graph ever waits on
 // another workgroup and the graph's tables stay in one XCD's L2.
+// Masks over the low parts L of a tile (tile-mask bit order: bit
+// 8 (L & 3) + (L >> 2) stands for L): s_m[0] = every L, s_m[1 + T] = the L
+// meeting T (none for T = 0), s_m[33 + y] = the L without low vertex y.
+constexpr int kLowMaskWords = 38;
+template <int LB>
+__device__ __forceinline__ void LowMasksShared(uint32_t* s_m) {
+  const int t = threadIdx.x;
+  if (t < 38) {
+    uint32_t m = 0;
+    for (int L = 0; L < (1 << LB); ++L) {
+      const bool take = t == 0 ? true : t < 33 ? (L & (t - 1)) != 0
+                                               : (t - 33 < LB && !((L >> (t - 33)) & 1));
+      if (take) m |= 1u << (8 * (L & 3) + (L >> 2));
+    }
+    s_m[t] = m;
+  }
+}
+
+// The N(H), isolation-by-high and isolation-by-low tables of graph g (the
+// nbh / isoh / isol arrays MakeGraph builds on the host), built in LDS from
+// g's adjacency rows: entry by entry the same unions and intersections over
+// the bits of each chunk.  s_m from LowMasksShared.
+template <int LB>
+__device__ __forceinline__ void BuildTables(const DPGraph* __restrict__ g, const uint32_t* s_m,
+                                            uint32_t* s_tab) {
+  const int nh = static_cast<int>(g->nh);
+  constexpr uint32_t kLow = (1u << LB) - 1u;
+  for (int i = threadIdx.x; i < kTabWords; i += blockDim.x) {
+    uint32_t m;
+    if (i < kTabIsoH) {  // N(chunk) = union of its vertices' rows
+      const int c = i >> 7;
+      m = 0;
+      for (uint32_t ch = i & 127; ch; ch &= ch - 1) {
+        const int x = 7 * c + __builtin_ctz(ch);
+        if (x < nh) m |= g->adj[LB + x];
+      }
+    } else if (i < kTabIsoL) {  // the L that give every x of the chunk a neighbour
+      const int j = i - kTabIsoH, c = j >> 7;
+      m = s_m[0];
+      for (uint32_t ch = j & 127; ch; ch &= ch - 1) {
+        const int x = 7 * c + __builtin_ctz(ch);
+        const uint32_t low = x < nh ? g->adj[LB + x] & kLow : 0u;
+        m &= low ? s_m[1 + low] : 0u;
+      }
+    } else {  // the L in which every y of Y n L has a neighbour in L
+      const uint32_t Y = static_cast<uint32_t>(i - kTabIsoL);
+      if (Y > kLow) {
+        m = 0;
+      } else {
+        m = s_m[0];
+        for (uint32_t yy = Y; yy; yy &= yy - 1) {
+          const int y = __builtin_ctz(yy);
+          const uint32_t low = g->adj[y] & kLow;
+          m &= s_m[33 + y] | (low ? s_m[1 + low] : 0u);
+        }
+      }
+    }
+    s_tab[i] = m;
+  }
+}
+
 template <int LB>
 __global__ __launch_bounds__(1024) void DPPersistentKernel(
     const DPGraph* __restrict__ gs, const uint32_t* __restrict__ g_cb,
@@ -530,15 +592,18 @@ __global__ __launch_bounds__(1024) void DPPersistentKernel(
     int R) {
   __shared__ uint32_t s_lvl[kLvlWords];
   __shared__ uint32_t s_tab[kTabWords];
+  __shared__ uint32_t s_masks[kLowMaskWords];
   __shared__ int s_gi;
   LoadPascal<LB>(g_cb, s_lvl);
+  LowMasksShared<LB>(s_masks);
   for (;;) {
     if (threadIdx.x == 0) s_gi = static_cast<int>(atomicAdd(queue, 1u));
     __syncthreads();
     const int gi = s_gi;
     if (gi >= ngraphs) break;  // uniform: every thread read the same s_gi
     const DPGraph* g = gs + gi;
-    LoadShared<LB>(g, g_cb, s_lvl, s_tab, false);
+    // only the record's head (up to nbh) comes from the host
+    BuildTables<LB>(g, s_masks, s_tab);
     __syncthreads();
     const int nh = static_cast<int>(g->nh);
     for (int h = 0; h <= nh; ++h) {
@@ -778,7 +843,8 @@ const LowMasks& LowMasksFor(int lb) {
 
 // Tile offsets of each high level, each level padded to a 256-byte boundary
 // so no cache line holds tiles of two levels.
-DPGraph MakeGraph(int n, const uint64_t* adj, int lb, uint64_t table_off, int cb = 0) {
+DPGraph MakeGraph(int n, const uint64_t* adj, int lb, uint64_t table_off, int cb = 0,
+                  bool tables = true) {
   DPGraph G;
   std::memset(&G, 0, sizeof(G));
   G.n = n;
@@ -801,6 +867,7 @@ DPGraph MakeGraph(int n, const uint64_t* adj, int lb, uint64_t table_off, int cb
       if (L >> l & 1) m |= G.adj[l];
     G.nl[L] = m;
   }
+  if (!tables) return G;  // the persistent kernel builds them in LDS
   // N(chunk) = N(chunk without its lowest bit) | N(lowest bit)
   for (int c = 0; c < 4; ++c) {
     G.nbh[128 * c] = 0;
@@ -1314,17 +1381,59 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
              "empty batch or null pointer");
   TD_REQUIRE(mode >= 0 && mode <= 2, TD_E_INVALID, "mode must be 0, 1 or 2");
   const int lb = ctx->dp.low_bits;
+  TD_REQUIRE(lb >= 2 && lb <= 5, TD_E_INVALID, "low_bits must be in [2, 5]");
   std::vector<size_t> row_off(ngraphs + 1, 0);
   for (int i = 0; i < ngraphs; ++i) {
     TD_REQUIRE(ns[i] >= 1 && ns[i] <= kMaxDPN, TD_E_INVALID,
                "batch graphs need n in [1, 32]");
     row_off[i + 1] = row_off[i] + ns[i];
   }
-  for (int i = 0; i < ngraphs; ++i) ValidateGraph(ns[i], adj_concat + row_off[i], kMaxDPN);
-  // graphs above kMaxBatchN vertices go through the single-graph kernels
-  std::vector<int> large;
-  for (int i = 0; i < ngraphs; ++i)
-    if (ns[i] > kMaxBatchN) large.push_back(i);
+  // graphs above kMaxBatchN vertices go through the single-graph kernels;
+  // the rest run largest first, so the persistent queue ends with short work
+  std::vector<int> large, perm;
+  for (int i = 0; i < ngraphs; ++i) (ns[i] > kMaxBatchN ? large : perm).push_back(i);
+  std::stable_sort(perm.begin(), perm.end(),
+                   [&](int a, int b) { return ns[a] > ns[b]; });
+  const int nsmall = static_cast<int>(perm.size());
+  // page-locked block of the small graphs' records, start sets and results
+  const size_t o_start = sizeof(DPGraph) * nsmall;
+  const size_t o_par = (o_start + 4ull * nsmall + 15) / 16 * 16;
+  const size_t o_td = o_par + 64ull * nsmall;
+  uint8_t* block = static_cast<uint8_t*>(ctx->Pinned(o_td + 4ull * nsmall + 16, 1));
+  DPGraph* Gs = reinterpret_cast<DPGraph*>(block);
+  // every graph is checked, and the small graphs' records are built, on up
+  // to 16 host threads (about 0.3 us and 1 us a graph) before any GPU work;
+  // a failure is reported for the lowest failing graph, as a serial check
+  // would
+  {
+    const int nthreads = static_cast<int>(std::min<unsigned>(
+        16u, std::max(1u, std::min(std::thread::hardware_concurrency(),
+                                   static_cast<unsigned>(ngraphs / 128)))));
+    std::atomic<int> bad{ngraphs};
+    auto work = [&](int t) {
+      for (int i = t; i < ngraphs; i += nthreads) {
+        try {
+          ValidateGraph(ns[i], adj_concat + row_off[i], kMaxDPN);
+        } catch (const Error&) {
+          int cur = bad.load();
+          while (i < cur && !bad.compare_exchange_weak(cur, i)) {
+          }
+        }
+      }
+      for (int j = t; j < nsmall; j += nthreads) {
+        const int i = perm[j];
+        Gs[j] = MakeGraph(ns[i], adj_concat + row_off[i], lb, 0, 0, mode != 0);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+    if (bad.load() < ngraphs) {
+      const int i = bad.load();
+      ValidateGraph(ns[i], adj_concat + row_off[i], kMaxDPN);  // throws its error
+    }
+  }
   uint64_t subsets = 0, alg = 0;
   for (int i = 0; i < ngraphs; ++i) {
     subsets += 1ull << ns[i];
@@ -1349,35 +1458,11 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
                           std::chrono::steady_clock::now() - t0)
                           .count();
   };
-  // the rest: largest first so the persistent queue ends with short work
-  std::vector<int> perm;
-  for (int i = 0; i < ngraphs; ++i)
-    if (ns[i] <= kMaxBatchN) perm.push_back(i);
-  std::stable_sort(perm.begin(), perm.end(),
-                   [&](int a, int b) { return ns[a] > ns[b]; });
-  if (perm.empty()) {
+  if (nsmall == 0) {
     finish();
     return;
   }
-  ngraphs = static_cast<int>(perm.size());
-  // the graph records go up from page-locked memory; they are built on
-  // several host threads (about 1 us each), then given their arena offsets
-  DPGraph* Gs = static_cast<DPGraph*>(ctx->Pinned(sizeof(DPGraph) * ngraphs));
-  {
-    const int nthreads = static_cast<int>(std::min<unsigned>(
-        16u, std::max(1u, std::min(std::thread::hardware_concurrency(),
-                                   static_cast<unsigned>(ngraphs / 128)))));
-    auto work = [&](int t) {
-      for (int j = t; j < ngraphs; j += nthreads) {
-        const int i = perm[j];
-        Gs[j] = MakeGraph(ns[i], adj_concat + row_off[i], lb, 0);
-      }
-    };
-    std::vector<std::thread> pool;
-    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
-    work(0);
-    for (auto& th : pool) th.join();
-  }
+  ngraphs = nsmall;
   uint64_t off = 0;
   for (int j = 0; j < ngraphs; ++j) {
     Gs[j].table = off;
@@ -1391,12 +1476,20 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
   uint32_t* dstart = static_cast<uint32_t*>(ctx->Buffer("dp.batch_start", 4 * ngraphs));
   int8_t* dpar = static_cast<int8_t*>(ctx->Buffer("dp.batch_parent", 64ull * ngraphs));
   int32_t* dtd = static_cast<int32_t*>(ctx->Buffer("dp.batch_td", 4ull * ngraphs));
-  std::vector<uint32_t> start(ngraphs);
+  uint32_t* start = reinterpret_cast<uint32_t*>(block + o_start);
   for (int j = 0; j < ngraphs; ++j)
     start[j] = Gs[j].n >= 32 ? 0xFFFFFFFFu : ((1u << Gs[j].n) - 1u);
-  TD_HIP(hipMemcpyAsync(dGs, Gs, sizeof(DPGraph) * ngraphs,
-                        hipMemcpyHostToDevice, ctx->stream));
-  TD_HIP(hipMemcpyAsync(dstart, start.data(), 4 * ngraphs, hipMemcpyHostToDevice, ctx->stream));
+  if (mode == 0) {
+    // the persistent kernel reads only each record's head and builds the
+    // tables itself: 416 of the 4640 bytes per graph go up
+    constexpr size_t kHead = offsetof(DPGraph, nbh);
+    TD_HIP(hipMemcpy2DAsync(dGs, sizeof(DPGraph), Gs, sizeof(DPGraph), kHead, ngraphs,
+                            hipMemcpyHostToDevice, ctx->stream));
+  } else {
+    TD_HIP(hipMemcpyAsync(dGs, Gs, sizeof(DPGraph) * ngraphs, hipMemcpyHostToDevice,
+                          ctx->stream));
+  }
+  TD_HIP(hipMemcpyAsync(dstart, start, 4 * ngraphs, hipMemcpyHostToDevice, ctx->stream));
   uint64_t small_launches = 0;
   TD_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
   switch (lb) {
@@ -1414,15 +1507,15 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
   TD_HIP(hipGetLastError());
   launches += small_launches + 1;
   TD_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-  std::vector<int8_t> par(64ull * ngraphs);
-  std::vector<int32_t> tds(ngraphs);
-  TD_HIP(hipMemcpyAsync(par.data(), dpar, par.size(), hipMemcpyDeviceToHost, ctx->stream));
-  TD_HIP(hipMemcpyAsync(tds.data(), dtd, 4ull * ngraphs, hipMemcpyDeviceToHost, ctx->stream));
+  int8_t* par = reinterpret_cast<int8_t*>(block + o_par);
+  int32_t* tds = reinterpret_cast<int32_t*>(block + o_td);
+  TD_HIP(hipMemcpyAsync(par, dpar, 64ull * ngraphs, hipMemcpyDeviceToHost, ctx->stream));
+  TD_HIP(hipMemcpyAsync(tds, dtd, 4ull * ngraphs, hipMemcpyDeviceToHost, ctx->stream));
   TD_HIP(hipStreamSynchronize(ctx->stream));
   for (int j = 0; j < ngraphs; ++j) {
     const int i = perm[j];
     if (td_out) td_out[i] = static_cast<uint8_t>(tds[j]);
-    if (parent_concat) std::memcpy(parent_concat + row_off[i], par.data() + 64ull * j, ns[i]);
+    if (parent_concat) std::memcpy(parent_concat + row_off[i], par + 64ull * j, ns[i]);
   }
   level_ms += Elapsed(ctx->ev[0], ctx->ev[1]);
   tree_ms += Elapsed(ctx->ev[1], ctx->ev[2]);

@@ -285,6 +285,26 @@ def test_batch_modes_equal_single(ctx, oracle):
         assert st.graphs == len(graphs)
 
 
+@pytest.mark.parametrize("low_bits", [2, 3, 4, 5])
+def test_persistent_batch_every_tile_width(ctx, oracle, low_bits):
+    # mode 0 builds each graph's neighbourhood and isolation tables in LDS
+    # from the adjacency rows (only the records' heads are uploaded)
+    rng = random.Random(20 + low_bits)
+    graphs = [G.gnp(rng.randint(1, 24), rng.uniform(0.1, 0.6), rng.getrandbits(32))
+              for _ in range(60)]
+    graphs += [G.complete(7), G.edgeless(9), G.path(23), G.star(12)]
+    want = [oracle.dp_fast(g) for g in graphs]
+    saved = ctx.get_param("dp.low_bits")
+    ctx.set_param("dp.low_bits", low_bits)
+    try:
+        print("persistent batch low_bits", low_bits, flush=True)
+        tds, pars, st = api.dp_solve_batch(ctx, graphs, 0)
+        for g, (td, par), t, p in zip(graphs, want, tds, pars):
+            assert (t, p) == (td, par), g.name
+    finally:
+        ctx.set_param("dp.low_bits", saved)
+
+
 @pytest.mark.parametrize("use_graph", [1, 0])
 def test_batch_with_large_graphs_equals_single(ctx, oracle, use_graph):
     # graphs above 26 vertices take the two-slot pipeline (levels on the
