@@ -2,10 +2,15 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
 #include <cstddef>
 #include <cstdint>
+#include <exception>
+#include <functional>
 #include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/tdhip.h"
@@ -111,6 +116,85 @@ struct DeviceBuffer {
 
 }  // namespace tdhip
 
+namespace tdhip {
+// Host worker threads kept for the context's lifetime: Run(f) calls f(t, T)
+// on T threads (the caller is thread 0) and returns when all are done.
+// Starting threads costs about 50 us each, more than the per-call work
+// they share out.
+class WorkerPool {
+ public:
+  ~WorkerPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+      ++gen_;
+    }
+    cv_.notify_all();
+    for (auto& th : threads_) th.join();
+  }
+  int Size() const { return static_cast<int>(threads_.size()) + 1; }
+  void Start(int n) {
+    if (!threads_.empty() || n <= 1) return;
+    for (int t = 1; t < n; ++t) threads_.emplace_back([this, t] { Loop(t); });
+  }
+  // f must not throw on the worker threads; an exception on the caller's
+  // share is rethrown after every worker has finished.
+  void Run(const std::function<void(int, int)>& f) {
+    const int T = Size();
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      job_ = &f;
+      pending_ = T - 1;
+      ++gen_;
+    }
+    cv_.notify_all();
+    std::exception_ptr err;
+    try {
+      f(0, T);
+    } catch (...) {
+      err = std::current_exception();
+    }
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      done_.wait(lk, [this] { return pending_ == 0; });
+      job_ = nullptr;
+    }
+    if (err) std::rethrow_exception(err);
+  }
+
+ private:
+  void Loop(int t) {
+    uint64_t seen = 0;
+    for (;;) {
+      const std::function<void(int, int)>* job;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return gen_ != seen; });
+        seen = gen_;
+        if (stop_) return;
+        job = job_;
+      }
+      try {
+        (*job)(t, Size());
+      } catch (...) {
+        // jobs catch their own errors; nothing may leave a worker thread
+      }
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (--pending_ == 0) done_.notify_one();
+      }
+    }
+  }
+  std::vector<std::thread> threads_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int, int)>* job_ = nullptr;
+  uint64_t gen_ = 0;
+  int pending_ = 0;
+  bool stop_ = false;
+};
+}  // namespace tdhip
+
 struct td_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -134,6 +218,7 @@ struct td_ctx {
   void* Pinned(size_t bytes, int slot = 0);
 
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  tdhip::WorkerPool workers;  // host threads for per-graph setup work
   // second stream and events of the two-slot pipeline that overlaps one
   // large graph's decomposition with the next graph's levels (created on
   // first use)

@@ -1401,17 +1401,14 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
   const size_t o_td = o_par + 64ull * nsmall;
   uint8_t* block = static_cast<uint8_t*>(ctx->Pinned(o_td + 4ull * nsmall + 16, 1));
   DPGraph* Gs = reinterpret_cast<DPGraph*>(block);
-  // every graph is checked, and the small graphs' records are built, on up
-  // to 16 host threads (about 0.3 us and 1 us a graph) before any GPU work;
+  // every graph is checked, and the small graphs' records are built, on the
+  // context's worker threads (about 0.3 us and 1 us a graph) before any GPU work;
   // a failure is reported for the lowest failing graph, as a serial check
   // would
   {
-    const int nthreads = static_cast<int>(std::min<unsigned>(
-        16u, std::max(1u, std::min(std::thread::hardware_concurrency(),
-                                   static_cast<unsigned>(ngraphs / 128)))));
     std::atomic<int> bad{ngraphs};
-    auto work = [&](int t) {
-      for (int i = t; i < ngraphs; i += nthreads) {
+    auto work = [&](int t, int T) {
+      for (int i = t; i < ngraphs; i += T) {
         try {
           ValidateGraph(ns[i], adj_concat + row_off[i], kMaxDPN);
         } catch (const Error&) {
@@ -1420,15 +1417,18 @@ void SolveBatch(td_ctx* ctx, int ngraphs, const int32_t* ns,
           }
         }
       }
-      for (int j = t; j < nsmall; j += nthreads) {
+      for (int j = t; j < nsmall; j += T) {
         const int i = perm[j];
         Gs[j] = MakeGraph(ns[i], adj_concat + row_off[i], lb, 0, 0, mode != 0);
       }
     };
-    std::vector<std::thread> pool;
-    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
-    work(0);
-    for (auto& th : pool) th.join();
+    if (ngraphs < 256) {
+      work(0, 1);
+    } else {
+      ctx->workers.Start(static_cast<int>(
+          std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()))));
+      ctx->workers.Run(work);
+    }
     if (bad.load() < ngraphs) {
       const int i = bad.load();
       ValidateGraph(ns[i], adj_concat + row_off[i], kMaxDPN);  // throws its error
