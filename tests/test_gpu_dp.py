@@ -402,6 +402,35 @@ def test_errors(ctx):
     assert e.value.code in (api.TD_E_STATE, api.TD_E_INVALID)
 
 
+def test_batch_errors_leave_context_usable(oracle):
+    # a bad graph inside a batch big enough for the worker threads: the
+    # lowest failing graph is reported and nothing runs on the GPU
+    ctx = api.Context()
+    rng = random.Random(3)
+    graphs = [G.gnp(rng.randint(6, 14), 0.3, rng.getrandbits(32)) for _ in range(300)]
+    broken = list(graphs)
+    broken[150] = G.Graph(3, [0b010, 0, 0], "asym")  # 0-1 listed only in row 0
+    broken[250] = G.Graph(2, [0b01, 0], "loop")
+    with pytest.raises(api.TDError) as e:
+        api.dp_solve_batch(ctx, broken, 0)
+    assert e.value.code == api.TD_E_INVALID
+    assert "asymmetric" in str(e.value)
+    tds, pars, _ = api.dp_solve_batch(ctx, graphs[:40], 0)
+    assert list(zip(tds, pars)) == [oracle.dp_fast(g) for g in graphs[:40]]
+    ctx.close()
+    # a budget with room for one n = 30 table but not the pipeline's second
+    small = api.Context(mem_budget_bytes=(1 << 30) + (256 << 20))
+    big = [G.gnp(30, 0.2, 1), G.gnp(30, 0.2, 2)]
+    print("batch budget", flush=True)
+    with pytest.raises(api.TDError) as e:
+        api.dp_solve_batch(small, big, 0)
+    assert e.value.code == api.TD_E_NOMEM
+    dp = api.DynamicGPU(small)
+    dp(G.gnp(20, 0.3, 5))
+    assert (dp.treedepth, dp.parent) == oracle.dp_fast(G.gnp(20, 0.3, 5))
+    small.close()
+
+
 def test_deterministic(dp):
     g = G.gnp(22, 0.35, 8)
     dp(g)
