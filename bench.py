@@ -46,6 +46,7 @@ import argparse
 import json
 import math
 import os
+import random
 import sys
 import time
 from pathlib import Path
@@ -620,6 +621,22 @@ def main():
                   file=sys.stderr, flush=True)
         agree = all(a == b for a, b, c, d in zip(answers[False][0], answers[True][0],
                                                    answers[False][1], answers[True][1]) if c and d)
+        # 256 dense small graphs (n in [20,24], p in [0.5,0.8]): the plain
+        # search against routing them to the subset DP (bnb.dp_max_n = 24)
+        rng = random.Random(17)
+        dense = [G.gnp(rng.randint(20, 24), rng.uniform(0.5, 0.8), rng.getrandbits(32))
+                 for _ in range(256)]
+        routing = {}
+        for dpn in (0, 24):
+            ctx.set_param("bnb.dp_max_n", dpn)
+            api.bnb_solve_batch(ctx, dense)
+            t = time.perf_counter()
+            _, _, cert_d, _ = api.bnb_solve_batch(ctx, dense)
+            wall_d = time.perf_counter() - t
+            routing["dp_routing" if dpn else "search_only"] = {
+                "certified": int(sum(cert_d)), "wall_s": wall_d,
+                "certified_graphs_per_s": sum(cert_d) / wall_d}
+        ctx.set_param("bnb.dp_max_n", 0)
         rate = {k: v[1] / v[0] for k, v in tot.items()}
         out = {
             "metric": "certified graphs/sec (learned-ordered B&B, 1024 graphs, n in [40,64])",
@@ -644,6 +661,7 @@ def main():
                                 "learned": tot[True][2] / tot[True][0]},
                 "certified_answers_agree": agree,
                 "proposal_training_s": train_s,
+                "dense_n20_24_256_graphs": routing,
             },
         }
     else:

@@ -1089,9 +1089,9 @@ int64_t Param(td_ctx* ctx, const char* k, int64_t dflt) {
   return it == ctx->params.end() ? dflt : it->second;
 }
 
-void SolveBnB(td_ctx* ctx, int ngraphs, const int32_t* ns, const uint64_t* adj_concat,
-              const int32_t* ub_seed, const float* scores, uint8_t* td_out,
-              int8_t* parent_concat, uint8_t* certified_out, td_bnb_stats* stats) {
+void SolveBnBSearch(td_ctx* ctx, int ngraphs, const int32_t* ns, const uint64_t* adj_concat,
+                    const int32_t* ub_seed, const float* scores, uint8_t* td_out,
+                    int8_t* parent_concat, uint8_t* certified_out, td_bnb_stats* stats) {
   const auto t0 = std::chrono::steady_clock::now();
   TD_REQUIRE(ngraphs >= 1 && ns && adj_concat, TD_E_INVALID, "empty batch or null pointer");
   std::vector<size_t> off(ngraphs + 1, 0);
@@ -1177,6 +1177,88 @@ void SolveBnB(td_ctx* ctx, int ngraphs, const int32_t* ns, const uint64_t* adj_c
     stats->nodes = nodes;
     stats->memo_hits = hits;
     stats->kernel_ms = ms;
+    stats->total_ms = std::chrono::duration<double, std::milli>(
+                          std::chrono::steady_clock::now() - t0)
+                          .count();
+  }
+}
+
+// Graphs of at most "bnb.dp_max_n" vertices (0, the default: none) are
+// answered by the batched subset DP (td_dp_solve_batch): exact, certified,
+// with the DP's canonical tree; the others go to the search.  The DP does
+// 2^n work however dense the graph is, where the search's refutation can
+// grow far faster on dense small graphs.
+void SolveBnB(td_ctx* ctx, int ngraphs, const int32_t* ns, const uint64_t* adj_concat,
+              const int32_t* ub_seed, const float* scores, uint8_t* td_out,
+              int8_t* parent_concat, uint8_t* certified_out, td_bnb_stats* stats) {
+  const int dp_max_n = static_cast<int>(Param(ctx, "bnb.dp_max_n", 0));
+  TD_REQUIRE(dp_max_n >= 0 && dp_max_n <= 32, TD_E_INVALID, "bnb.dp_max_n in [0, 32]");
+  TD_REQUIRE(ngraphs >= 1 && ns && adj_concat, TD_E_INVALID, "empty batch or null pointer");
+  std::vector<int> dpi, rest;
+  for (int i = 0; i < ngraphs; ++i) {
+    TD_REQUIRE(ns[i] >= 1 && ns[i] <= kMaxBnBN, TD_E_INVALID, "n must be in [1, 64]");
+    (ns[i] <= dp_max_n ? dpi : rest).push_back(i);
+  }
+  if (dpi.empty()) {
+    SolveBnBSearch(ctx, ngraphs, ns, adj_concat, ub_seed, scores, td_out, parent_concat,
+                   certified_out, stats);
+    return;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<size_t> off(ngraphs + 1, 0);
+  for (int i = 0; i < ngraphs; ++i) off[i + 1] = off[i] + ns[i];
+  // one side's graphs packed in input order; results scattered back
+  struct Part {
+    std::vector<int32_t> n, seed;
+    std::vector<uint64_t> adj;
+    std::vector<float> score;
+    std::vector<uint8_t> td, cert;
+    std::vector<int8_t> par;
+  };
+  auto pack = [&](const std::vector<int>& idx, Part& P) {
+    for (int i : idx) {
+      P.n.push_back(ns[i]);
+      P.seed.push_back(ub_seed ? ub_seed[i] : 0);
+      P.adj.insert(P.adj.end(), adj_concat + off[i], adj_concat + off[i + 1]);
+      if (scores) P.score.insert(P.score.end(), scores + off[i], scores + off[i + 1]);
+    }
+    P.td.assign(idx.size(), 0);
+    P.cert.assign(idx.size(), 0);
+    P.par.assign(P.adj.size(), 0);
+  };
+  Part D, B;
+  pack(dpi, D);
+  pack(rest, B);
+  td_dp_stats dst{};
+  const int rc = td_dp_solve_batch(ctx, static_cast<int32_t>(dpi.size()), D.n.data(),
+                                   D.adj.data(), D.td.data(), D.par.data(), 0, &dst);
+  TD_REQUIRE(rc == TD_OK, rc, ctx->err);
+  td_bnb_stats bst{};
+  if (!rest.empty())
+    SolveBnBSearch(ctx, static_cast<int>(rest.size()), B.n.data(), B.adj.data(),
+                   ub_seed ? B.seed.data() : nullptr, scores ? B.score.data() : nullptr,
+                   B.td.data(), B.par.data(), B.cert.data(), &bst);
+  const std::vector<td_ctx::BnBDetail> bdet = ctx->bnb_last;  // the search's graphs
+  ctx->bnb_last.assign(ngraphs, {0, 0, 0});
+  auto scatter = [&](const std::vector<int>& idx, const Part& P, bool exact) {
+    size_t r = 0;
+    for (size_t j = 0; j < idx.size(); ++j) {
+      const int i = idx[j];
+      if (td_out) td_out[i] = P.td[j];
+      if (certified_out) certified_out[i] = exact ? 1 : P.cert[j];
+      if (parent_concat) std::memcpy(parent_concat + off[i], P.par.data() + r, ns[i]);
+      r += ns[i];
+      ctx->bnb_last[i] = exact ? td_ctx::BnBDetail{P.td[j], P.td[j], 0} : bdet[j];
+    }
+  };
+  scatter(dpi, D, true);
+  scatter(rest, B, false);
+  if (stats) {
+    stats->graphs = static_cast<uint64_t>(ngraphs);
+    stats->certified = dpi.size() + bst.certified;
+    stats->nodes = bst.nodes;
+    stats->memo_hits = bst.memo_hits;
+    stats->kernel_ms = bst.kernel_ms + dst.level_ms + dst.tree_ms;
     stats->total_ms = std::chrono::duration<double, std::milli>(
                           std::chrono::steady_clock::now() - t0)
                           .count();

@@ -225,7 +225,13 @@ int td_bnb_solve(td_ctx* ctx, int32_t n, const uint64_t* adj, int32_t heur_td,
  * heur_td.  vertex_scores (nullable, sum n[i] floats): a learned proposal;
  * higher score = tried first as a root and used for an extra greedy upper
  * bound.  Scores only order the search; they never prune.
- * td_out / certified_out: ngraphs bytes; parent_concat: sum n[i] bytes. */
+ * td_out / certified_out: ngraphs bytes; parent_concat: sum n[i] bytes.
+ * With the context parameter "bnb.dp_max_n" = m > 0 (default 0), graphs of
+ * at most m <= 32 vertices are answered by td_dp_solve_batch instead
+ * (certified, td_dp_solve's tree; their details read heuristic = lower
+ * bound = td, 0 nodes), the DP+B&B hybrid the reference proposes but does
+ * not build (DOC/our_approach/main.tex:181).  td_bnb_solve takes the same
+ * route. */
 int td_bnb_solve_batch(td_ctx* ctx, int32_t ngraphs, const int32_t* n,
                        const uint64_t* adj_concat, const int32_t* ub_seed,
                        const float* vertex_scores, uint8_t* td_out,

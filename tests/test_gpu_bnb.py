@@ -174,3 +174,45 @@ def test_bnb_closed_forms_at_the_size_limit(ctx, oracle):
         print("closed form", g.name, g.n, want, t, c, flush=True)
         assert c and t == want, g.name
         assert oracle.certify(g, p, t), g.name
+
+
+def test_dp_routing_of_small_graphs(ctx, oracle):
+    # bnb.dp_max_n: graphs up to that size are answered by the batched subset
+    # DP (certified, the DP's canonical tree); larger ones by the search,
+    # with the same answers as without the routing
+    rng = random.Random(5)
+    dense = [G.gnp(rng.randint(14, 22), rng.uniform(0.5, 0.8), rng.getrandbits(32))
+             for _ in range(12)]
+    large = G.workload_bnb(32, 3)[16:24]  # grids and trees + edges, n >= 40
+    gs = []
+    for a, b in zip(dense, large):
+        gs += [a, b]
+    gs += dense[len(large):]
+    ctx.set_param("bnb.node_budget", 50000)
+    try:
+        print("bnb plain", flush=True)
+        td0, _, c0, _ = api.bnb_solve_batch(ctx, gs)
+        ctx.set_param("bnb.dp_max_n", 24)
+        print("bnb with dp routing", flush=True)
+        td1, par1, c1, st = api.bnb_solve_batch(ctx, gs)
+        for g, t0_, cc0, t1, p1, cc1 in zip(gs, td0, c0, td1, par1, c1):
+            if g.n <= 24:
+                assert cc1, g.name
+                assert (t1, p1) == oracle.dp_fast(g), g.name
+                if cc0:
+                    assert t0_ == t1, g.name
+            else:
+                assert (t1, cc1) == (t0_, cc0), g.name
+                assert oracle.certify(g, p1, t1), g.name
+        assert st.certified == sum(c1)
+        heur, lb, nodes = api.bnb_last_details(ctx, len(gs))
+        for g, t, h, l, nd in zip(gs, td1, heur, lb, nodes):
+            if g.n <= 24:
+                assert (h, l, nd) == (t, t, 0), g.name
+        # the single-graph entry takes the same route
+        bnb = api.BnBGPU(ctx)
+        r, certified = bnb.solve(dense[0])
+        assert certified and r.treedepth == oracle.dp_fast(dense[0])[0]
+    finally:
+        ctx.set_param("bnb.dp_max_n", 0)
+        ctx.set_param("bnb.node_budget", 2000000)
