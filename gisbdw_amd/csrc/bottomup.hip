@@ -475,20 +475,22 @@ int td_bottom_up(td_ctx* ctx, int32_t n, const uint64_t* adj, int32_t iterations
                          height + off[k], dmaxh);
       TD_HIP(hipGetLastError());
     }
-    TD_HIP(hipStreamSynchronize(ctx->stream));
     ctx->bu_n = n;
     ctx->bu_iters = K;
     ctx->bu_off = off;
-    ctx->have_bu = true;
     if (iterations_out) *iterations_out = K;
     if (K < n + 1) {  // BottomUpHeuristicGPU::Get returns UINT_MAX treedepth
+      TD_HIP(hipStreamSynchronize(ctx->stream));
+      ctx->have_bu = true;
       *treedepth_out = -1;
       return;
     }
     // GetElimination(n, n, 0), then the elimination forest of that order
-    // (EliminationTree::Eliminate ... Decompose, bottom_up_heuristic.hpp:23-28)
+    // (EliminationTree::Eliminate ... Decompose, bottom_up_heuristic.hpp:23-28);
+    // the walk follows the levels on the same stream and synchronizes
     const uint32_t full = n == 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
     const std::vector<int32_t> order = BuWalk(ctx, full, n);
+    ctx->have_bu = true;
     std::vector<int> parent(n, -1);
     std::vector<char> added(n, 0);
     for (int i = n - 1; i >= 0; --i) {
