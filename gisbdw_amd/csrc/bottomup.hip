@@ -17,14 +17,16 @@
 // the whole forest for w outside S.  Adding v to S \ v then needs one record:
 // slot v gives the height of S \ v, v's neighbours' slots the heights that v
 // joins (the new component's height is 1 + their largest, 1 if none), and the
-// forest height is the larger of the two.  A level's heights are at most k and
-// at most 1 + the largest height of the level below, so a level whose heights
-// must stay below 16 stores 4-bit slots (16-byte records), any other level
-// bytes (32-byte records), instead of the reference's n + 2 union-find bytes;
-// each launch finds the width from the largest heights the launches before it
-// recorded.  The history keeps, per subset, the vertex added last and the
-// height (history_ in the reference), 2 bytes per subset over all levels; the
-// records live for two levels only.
+// forest height is the larger of the two.  Records are bit-sliced: plane j is
+// a 32-bit mask holding bit j of every slot, so the largest slot over v's
+// neighbours is a descent over the planes with no per-neighbour loop.  A
+// level's heights are at most k and at most 1 + the largest height of the
+// level below, so a level whose heights must stay below 16 stores 4 planes
+// (16-byte records), any other level 8 (32 bytes), instead of the reference's
+// n + 2 union-find bytes; each launch finds the width from the largest heights
+// the launches before it recorded.  The history keeps, per subset, the vertex
+// added last and the height (history_ in the reference), 2 bytes per subset
+// over all levels; the records live for two levels only.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -36,7 +38,7 @@ namespace {
 
 constexpr int kBuStride = 33;  // s_cb[x * 33 + j] = C(x, j)
 constexpr int kRecBytes = 32;  // largest per-subset record (byte slots)
-constexpr int kNibLevels = 15; // levels k <= 15 always fit 4-bit slots
+constexpr int kFourPlaneLevels = 15; // levels k <= 15 always fit four planes
 constexpr int kBuBlock = 256;
 // The binomial buffer continues with O_k = sum_{j<k} C(16, j) (k = 0..16)
 // and, from word kBuTailWord, the colex k-subsets of {0..15} as 16-bit masks
@@ -75,99 +77,77 @@ __device__ __forceinline__ uint32_t BuComponent(const uint32_t* s_adj, uint32_t 
   return comp;
 }
 
-// A record in registers: a, b (4-bit slots, slot w at bits 4w of the
-// 128-bit value b:a) or a..d (byte slots, slot w at bits 8w of d:c:b:a).
-// Named fields, not an array, so that slot selects stay register selects.
+// A record in registers, bit-sliced: plane j (p0..p7) holds bit j of every
+// slot, slot w at bit w of each plane.  Levels whose heights stay below 16
+// use planes 0-3 (16 bytes), the others planes 0-7 (32 bytes).  Named fields,
+// not an array, so that plane selects stay in registers.
 struct BuRec {
-  uint64_t a, b, c, d;
+  uint32_t p0, p1, p2, p3, p4, p5, p6, p7;
 };
 
-template <bool kNib>
-__device__ __forceinline__ BuRec BuLoad(const uint8_t* p) {
+template <int P>
+__device__ __forceinline__ BuRec BuLoad(const uint8_t* ptr) {
   BuRec q;
-  const uint4 x = reinterpret_cast<const uint4*>(p)[0];
-  q.a = x.x | static_cast<uint64_t>(x.y) << 32;
-  q.b = x.z | static_cast<uint64_t>(x.w) << 32;
-  q.c = q.d = 0;
-  if (!kNib) {
-    const uint4 y = reinterpret_cast<const uint4*>(p)[1];
-    q.c = y.x | static_cast<uint64_t>(y.y) << 32;
-    q.d = y.z | static_cast<uint64_t>(y.w) << 32;
+  const uint4 x = reinterpret_cast<const uint4*>(ptr)[0];
+  q.p0 = x.x;
+  q.p1 = x.y;
+  q.p2 = x.z;
+  q.p3 = x.w;
+  q.p4 = q.p5 = q.p6 = q.p7 = 0;
+  if (P == 8) {
+    const uint4 y = reinterpret_cast<const uint4*>(ptr)[1];
+    q.p4 = y.x;
+    q.p5 = y.y;
+    q.p6 = y.z;
+    q.p7 = y.w;
   }
   return q;
 }
 
-// Slot w; the words come as values (a reference would let the compiler turn
-// the selects into a dynamically indexed load from private memory).
-template <bool kNib>
-__device__ __forceinline__ int BuSlot(uint64_t a, uint64_t b, uint64_t c, uint64_t d, int w) {
-  if (kNib) {
-    const uint64_t x = (w & 16) ? b : a;
-    return static_cast<int>(x >> (4 * (w & 15))) & 15;
-  }
-  const uint64_t lo = (w & 8) ? b : a;
-  const uint64_t hi = (w & 8) ? d : c;
-  return static_cast<int>(((w & 16) ? hi : lo) >> (8 * (w & 7))) & 255;
+__device__ __forceinline__ uint32_t BuPlane(const BuRec& q, int j) {  // j compile-time
+  return j == 0 ? q.p0 : j == 1 ? q.p1 : j == 2 ? q.p2 : j == 3 ? q.p3
+       : j == 4 ? q.p4 : j == 5 ? q.p5 : j == 6 ? q.p6 : q.p7;
 }
 
-// 32-bit word d (compile-time after unrolling) of a record
-__device__ __forceinline__ uint32_t BuWord(uint64_t a, uint64_t b, uint64_t c, uint64_t d,
-                                           int i) {
-  const uint64_t x = i < 2 ? a : i < 4 ? b : i < 6 ? c : d;
-  return static_cast<uint32_t>(x >> (32 * (i & 1)));
-}
-
-// bit i of the low 8 bits -> 0xF in nibble i (fills the LDS table)
-__device__ __forceinline__ uint32_t BuSpreadNib(uint32_t x) {
-  x &= 0xFFu;
-  x = (x | (x << 12)) & 0x000F000Fu;
-  x = (x | (x << 6)) & 0x03030303u;
-  x = (x | (x << 3)) & 0x11111111u;
-  return x * 15u;
-}
-
-// bit i of the low 4 bits -> 0xFF in byte i
-__device__ __forceinline__ uint32_t BuSpreadByte(uint32_t x) {
-  x &= 0xFu;
-  x = (x | (x << 14)) & 0x00030003u;
-  x = (x | (x << 7)) & 0x01010101u;
-  return x * 255u;
-}
-
-// 8 nibbles -> 8 bytes (nibble i to byte i)
-__device__ __forceinline__ uint64_t BuNibToBytes(uint32_t v) {
-  uint64_t x = v;
-  x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
-  x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
-  x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+// Slot w.
+template <int P>
+__device__ __forceinline__ int BuSlot(const BuRec& q, int w) {
+  int x = 0;
+#pragma unroll
+  for (int j = 0; j < P; ++j) x |= static_cast<int>((BuPlane(q, j) >> w) & 1u) << j;
   return x;
 }
 
-// 8 bytes, each below 16 -> 8 nibbles (byte i to nibble i)
-__device__ __forceinline__ uint32_t BuBytesToNib(uint64_t x) {
-  x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
-  x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
-  x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
-  return static_cast<uint32_t>(x);
+// Largest slot among the vertices of M (0 for an empty M), from the top
+// plane down: keep the candidates that have the bit whenever any does.
+template <int P>
+__device__ __forceinline__ int BuMaxSlot(const BuRec& q, uint32_t M) {
+  int top = 0;
+#pragma unroll
+  for (int j = P - 1; j >= 0; --j) {
+    const uint32_t t = M & BuPlane(q, j);
+    top |= (t != 0u) << j;
+    M = t ? t : M;
+  }
+  return top;
 }
 
-// Levels whose heights all stay below 16 keep 4-bit slots: level k's
+// Levels whose heights all stay below 16 keep four planes: level k's
 // heights are at most 1 + the largest height of level k - 1 (and at most k).
-__device__ __forceinline__ bool BuNibLevel(int k, const uint32_t* maxh) {
-  return k <= kNibLevels || maxh[k - 1] <= 14u;
+__device__ __forceinline__ bool BuFourPlanes(int k, const uint32_t* maxh) {
+  return k <= kFourPlaneLevels || maxh[k - 1] <= 14u;
 }
 
 // Every k-subset of rank r < count at one level; returns the largest height
 // this lane wrote.
-template <bool kNibIn, bool kNibOut>
+template <int PIn, int POut>
 __device__ __forceinline__ uint32_t BuLevel(const uint32_t* s_cb, const uint32_t* s_adj,
-                                            const uint32_t* s_spread,
                                             const uint16_t* __restrict__ tail, int n, int k,
                                             uint32_t count, const uint8_t* __restrict__ prev_rec,
                                             uint8_t* __restrict__ cur_rec,
                                             int8_t* __restrict__ cur_last,
                                             uint8_t* __restrict__ cur_height) {
-  constexpr int kIn = kNibIn ? 16 : 32, kOut = kNibOut ? 16 : 32;
+  constexpr int kIn = 4 * PIn, kOut = 4 * POut;
   uint32_t lane_max = 0;
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < count;
        r += gridDim.x * blockDim.x) {
@@ -179,7 +159,7 @@ __device__ __forceinline__ uint32_t BuLevel(const uint32_t* s_cb, const uint32_t
     // strict "<" of the reference does.  The records of kGroup removals are
     // loaded together; every lane has k elements, so the groups (and the
     // `ok` flags) are the same across the wave.
-    constexpr int kGroup = kNibIn ? 4 : 2;
+    constexpr int kGroup = PIn == 4 ? 4 : 2;
     uint32_t sufa = 0, sufb = 0, brank = 0;
     int best = k + 1, bv = 0, bnew = 0;
     int i = k - 1;
@@ -204,20 +184,17 @@ __device__ __forceinline__ uint32_t BuLevel(const uint32_t* s_cb, const uint32_t
       BuRec qs[kGroup];
 #pragma unroll
       for (int u = 0; u < kGroup; ++u)
-        qs[u] = BuLoad<kNibIn>(prev_rec + static_cast<size_t>(r2s[u]) * kIn);
+        qs[u] = BuLoad<PIn>(prev_rec + static_cast<size_t>(r2s[u]) * kIn);
 #pragma unroll
       for (int u = 0; u < kGroup; ++u) {
         if (!ok[u]) break;
         const int v = vs[u];
-        const BuRec& q = qs[u];
-        int top = 0;
-        for (uint32_t a = s_adj[v] & S; a; a &= a - 1)
-          top = max(top, BuSlot<kNibIn>(q.a, q.b, q.c, q.d, __builtin_ctz(a)));
-        const int newh = top + 1;
+        // the joined component's height: 1 + the largest neighbour slot
+        const int newh = BuMaxSlot<PIn>(qs[u], s_adj[v] & S) + 1;
         // slot v of S \ v holds its forest height (0 for the empty set,
         // where the reference's union-find state holds 1; newh >= 1 makes
         // the two agree here)
-        const int h = max(BuSlot<kNibIn>(q.a, q.b, q.c, q.d, v), newh);
+        const int h = max(BuSlot<PIn>(qs[u], v), newh);
         if (h <= best) {
           best = h;
           bv = v;
@@ -227,44 +204,20 @@ __device__ __forceinline__ uint32_t BuLevel(const uint32_t* s_cb, const uint32_t
       }
     }
     const uint32_t comp = BuComponent(s_adj, S, bv);
-    BuRec q = BuLoad<kNibIn>(prev_rec + static_cast<size_t>(brank) * kIn);
-    if (kNibIn && !kNibOut) {  // widen 4-bit slots to bytes
-      const uint64_t lo = q.a, hi = q.b;
-      q.a = BuNibToBytes(static_cast<uint32_t>(lo));
-      q.b = BuNibToBytes(static_cast<uint32_t>(lo >> 32));
-      q.c = BuNibToBytes(static_cast<uint32_t>(hi));
-      q.d = BuNibToBytes(static_cast<uint32_t>(hi >> 32));
-    } else if (!kNibIn && kNibOut) {  // narrow bytes (all below 15 here)
-      q.a = BuBytesToNib(q.a) | static_cast<uint64_t>(BuBytesToNib(q.b)) << 32;
-      q.b = BuBytesToNib(q.c) | static_cast<uint64_t>(BuBytesToNib(q.d)) << 32;
-    }
+    // a level with four planes holds no height above 15, so the planes a
+    // narrower record lacks are zero and those a wider one drops are too
+    const BuRec q = BuLoad<PIn>(prev_rec + static_cast<size_t>(brank) * kIn);
     // kept slots: S outside v's component; v's component: bnew; outside S:
     // the forest height
-    const uint32_t out_of_s = ~S;
+    const uint32_t keep = S & ~comp, out_of_s = ~S;
     uint32_t w[8];
-    if (kNibOut) {
-      const uint32_t rep_new = static_cast<uint32_t>(bnew) * 0x11111111u;
-      const uint32_t rep_best = static_cast<uint32_t>(best) * 0x11111111u;
 #pragma unroll
-      for (int d = 0; d < 4; ++d) {
-        const uint32_t in = BuWord(q.a, q.b, q.c, q.d, d);
-        const uint32_t cm = s_spread[(comp >> (8 * d)) & 0xFFu];
-        const uint32_t nm = s_spread[(out_of_s >> (8 * d)) & 0xFFu];
-        w[d] = (in & ~(cm | nm)) | (rep_new & cm) | (rep_best & nm);
-      }
-    } else {
-      const uint32_t rep_new = static_cast<uint32_t>(bnew) * 0x01010101u;
-      const uint32_t rep_best = static_cast<uint32_t>(best) * 0x01010101u;
-#pragma unroll
-      for (int d = 0; d < 8; ++d) {
-        const uint32_t in = BuWord(q.a, q.b, q.c, q.d, d);
-        const uint32_t cm = BuSpreadByte(comp >> (4 * d)), nm = BuSpreadByte(out_of_s >> (4 * d));
-        w[d] = (in & ~(cm | nm)) | (rep_new & cm) | (rep_best & nm);
-      }
-    }
+    for (int j = 0; j < POut; ++j)
+      w[j] = (BuPlane(q, j) & keep) | ((bnew >> j) & 1 ? comp : 0u) |
+             ((best >> j) & 1 ? out_of_s : 0u);
     uint4* dst = reinterpret_cast<uint4*>(cur_rec + static_cast<size_t>(r) * kOut);
     dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    if (!kNibOut) dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    if (POut == 8) dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
     cur_last[r] = static_cast<int8_t>(bv);
     cur_height[r] = static_cast<uint8_t>(best);
     lane_max = max(lane_max, static_cast<uint32_t>(best));
@@ -273,8 +226,8 @@ __device__ __forceinline__ uint32_t BuLevel(const uint32_t* s_cb, const uint32_t
 }
 
 // One level k (1 <= k <= n).  maxh[j] is the largest height of level j
-// (maxh[0] = 0, maxh[k] cleared before the launch); it picks the slot width
-// of level k - 1's records and of this level's, and gets level k's maximum.
+// (maxh[0] = 0, maxh[k] cleared before the launch); it picks the width of
+// level k - 1's records and of this level's, and gets level k's maximum.
 __global__ __launch_bounds__(kBuBlock) void BottomUpLevelKernel(
     const uint32_t* __restrict__ g_adj, const uint32_t* __restrict__ g_cb, int n, int k,
     uint32_t count, const uint8_t* __restrict__ prev_rec, uint8_t* __restrict__ cur_rec,
@@ -282,28 +235,26 @@ __global__ __launch_bounds__(kBuBlock) void BottomUpLevelKernel(
     uint32_t* __restrict__ maxh) {
   __shared__ uint32_t s_cb[kBuCbWords];
   __shared__ uint32_t s_adj[32];
-  __shared__ uint32_t s_spread[256];
   __shared__ uint32_t s_max;
   for (int i = threadIdx.x; i < kBuCbWords; i += blockDim.x) s_cb[i] = g_cb[i];
-  for (int i = threadIdx.x; i < 256; i += blockDim.x) s_spread[i] = BuSpreadNib(i);
   if (threadIdx.x < 32) s_adj[threadIdx.x] = threadIdx.x < static_cast<unsigned>(n) ? g_adj[threadIdx.x] : 0u;
   if (threadIdx.x == 0) s_max = 0;
   __syncthreads();
   const uint16_t* tail = reinterpret_cast<const uint16_t*>(g_cb + kBuTailWord);
-  const bool nib_in = BuNibLevel(k - 1, maxh), nib_out = BuNibLevel(k, maxh);
+  const bool four_in = BuFourPlanes(k - 1, maxh), four_out = BuFourPlanes(k, maxh);
   uint32_t lane_max;
-  if (nib_in && nib_out)
-    lane_max = BuLevel<true, true>(s_cb, s_adj, s_spread, tail, n, k, count, prev_rec, cur_rec,
-                                   cur_last, cur_height);
-  else if (nib_in)
-    lane_max = BuLevel<true, false>(s_cb, s_adj, s_spread, tail, n, k, count, prev_rec,
-                                    cur_rec, cur_last, cur_height);
-  else if (nib_out)
-    lane_max = BuLevel<false, true>(s_cb, s_adj, s_spread, tail, n, k, count, prev_rec,
-                                    cur_rec, cur_last, cur_height);
+  if (four_in && four_out)
+    lane_max = BuLevel<4, 4>(s_cb, s_adj, tail, n, k, count, prev_rec, cur_rec, cur_last,
+                             cur_height);
+  else if (four_in)
+    lane_max = BuLevel<4, 8>(s_cb, s_adj, tail, n, k, count, prev_rec, cur_rec, cur_last,
+                             cur_height);
+  else if (four_out)
+    lane_max = BuLevel<8, 4>(s_cb, s_adj, tail, n, k, count, prev_rec, cur_rec, cur_last,
+                             cur_height);
   else
-    lane_max = BuLevel<false, false>(s_cb, s_adj, s_spread, tail, n, k, count, prev_rec,
-                                     cur_rec, cur_last, cur_height);
+    lane_max = BuLevel<8, 8>(s_cb, s_adj, tail, n, k, count, prev_rec, cur_rec, cur_last,
+                             cur_height);
   atomicMax(&s_max, lane_max);
   __syncthreads();
   if (threadIdx.x == 0) atomicMax(&maxh[k], s_max);
